@@ -1,0 +1,359 @@
+"""gpu_msm_amd — BLS12-446 G1/G2 multi-scalar multiplication on MI355X (gfx950).
+
+Python host mirror of the reference's Rust-facing API (reference
+rust-api/src/types.rs: G1Affine, G1Projective::msm, G2Affine,
+G2Projective::msm) over the library's C ABI (include/rust_wrapper.h,
+include/gpu_msm_ext.h), loaded with ctypes from the in-tree
+``libgpu_msm_amd.so``. There is no CPU fallback: if the HIP library is not
+built, every call raises.
+
+Arrays of points use the ABI record layouts as numpy structured dtypes
+(``G1_DTYPE`` 120 B, ``G2_DTYPE`` 232 B, ``G1P_DTYPE`` 168 B, ``G2P_DTYPE`` 336 B).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgpu_msm_amd.so")
+
+G1_DTYPE = np.dtype([("x", "<u8", 7), ("y", "<u8", 7), ("inf", "u1"), ("pad", "u1", 7)])
+G2_DTYPE = np.dtype([("x", "<u8", 14), ("y", "<u8", 14), ("inf", "u1"), ("pad", "u1", 7)])
+G1P_DTYPE = np.dtype([("X", "<u8", 7), ("Y", "<u8", 7), ("Z", "<u8", 7)])
+G2P_DTYPE = np.dtype([("X", "<u8", 14), ("Y", "<u8", 14), ("Z", "<u8", 14)])
+
+CURVE_G1 = 1
+CURVE_G2 = 2
+POINTS_MONTGOMERY = 0x1
+SCALARS_LT_R = 0x2
+R_ORDER = 0x511B70539F27995B34995830FA4D04C98CCC4C050BC7BB9B0E8D8CA34610428001400040001
+R_BITS = 299
+
+ERRORS = {
+    -1: "invalid size or argument",
+    -2: "device or stream setup failed",
+    -3: "device allocation failed",
+    -4: "HIP error during conversion or MSM",
+    -5: "scalar exceeds the declared bit bound",
+}
+
+# Exported C symbols (include/rust_wrapper.h + include/gpu_msm_ext.h).
+EXPORTS = [
+    "affine_to_projective_g1_wrapper", "affine_to_projective_g2_wrapper",
+    "projective_to_affine_g1_wrapper", "projective_to_affine_g2_wrapper",
+    "g1_point_at_infinity_wrapper", "g2_point_at_infinity_wrapper",
+    "g1_projective_point_at_infinity_wrapper", "g2_projective_point_at_infinity_wrapper",
+    "g1_is_infinity_wrapper", "g2_is_infinity_wrapper",
+    "g1_msm_wrapper", "g2_msm_wrapper",
+    "g1_from_montgomery_wrapper", "g2_from_montgomery_wrapper", "fp_to_montgomery_wrapper",
+    "g1_msm_multi_wrapper", "g2_msm_multi_wrapper", "gm_plan_query",
+    "gm_bases_upload", "gm_bases_from_k", "gm_bases_download", "gm_bases_count", "gm_bases_free",
+    "gm_scalars_upload", "gm_scalars_free", "gm_msm_resident",
+    "gm_set_profiling", "gm_profile_last",
+]
+
+
+class MsmError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise MsmError(f"HIP library not built: {LIB_PATH} (run python -m gpu_msm_amd.build)")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i, u32, sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32, ctypes.c_size_t
+    sig = {
+        "affine_to_projective_g1_wrapper": ([vp, vp], None),
+        "affine_to_projective_g2_wrapper": ([vp, vp], None),
+        "projective_to_affine_g1_wrapper": ([vp, vp], None),
+        "projective_to_affine_g2_wrapper": ([vp, vp], None),
+        "g1_point_at_infinity_wrapper": ([vp], None),
+        "g2_point_at_infinity_wrapper": ([vp], None),
+        "g1_projective_point_at_infinity_wrapper": ([vp], None),
+        "g2_projective_point_at_infinity_wrapper": ([vp], None),
+        "g1_is_infinity_wrapper": ([vp], ctypes.c_bool),
+        "g2_is_infinity_wrapper": ([vp], ctypes.c_bool),
+        "g1_msm_wrapper": ([vp, vp, vp, i, u32], i),
+        "g2_msm_wrapper": ([vp, vp, vp, i, u32], i),
+        "g1_from_montgomery_wrapper": ([vp, vp], None),
+        "g2_from_montgomery_wrapper": ([vp, vp], None),
+        "fp_to_montgomery_wrapper": ([vp, vp], None),
+        "g1_msm_multi_wrapper": ([vp, vp, vp, i, i, u32, u32], i),
+        "g2_msm_multi_wrapper": ([vp, vp, vp, i, i, u32, u32], i),
+        "gm_plan_query": ([i, i, i, i, i, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(sz)], i),
+        "gm_bases_upload": ([ctypes.POINTER(vp), i, vp, i, i, u32], i),
+        "gm_bases_from_k": ([ctypes.POINTER(vp), i, vp, i, u32], i),
+        "gm_bases_download": ([vp, vp, i, i], i),
+        "gm_bases_count": ([vp], i),
+        "gm_bases_free": ([vp], None),
+        "gm_scalars_upload": ([ctypes.POINTER(vp), vp, i, i, u32], i),
+        "gm_scalars_free": ([vp], None),
+        "gm_msm_resident": ([vp, vp, i, vp, i, i, i, i, i], i),
+        "gm_set_profiling": ([u32, i], i),
+        "gm_profile_last": ([u32, ctypes.POINTER(ctypes.c_float), i], i),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+def _ptr(a: np.ndarray) -> int:
+    if not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("array must be C-contiguous")
+    return a.ctypes.data
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != 0:
+        raise MsmError(f"{what} failed with error code {rc}: {ERRORS.get(rc, 'unknown')}")
+
+
+def int_to_limbs(v: int, n: int = 7) -> np.ndarray:
+    return np.array([(v >> (64 * k)) & 0xFFFFFFFFFFFFFFFF for k in range(n)], dtype=np.uint64)
+
+
+def limbs_to_int(a) -> int:
+    return sum(int(x) << (64 * k) for k, x in enumerate(np.asarray(a, dtype=np.uint64).ravel()))
+
+
+def scalars_to_limbs(scalars: Sequence[int], limbs: int = 5) -> np.ndarray:
+    """Python ints -> (n, limbs) little-endian u64 array."""
+    out = np.zeros((len(scalars), limbs), dtype=np.uint64)
+    for i, s in enumerate(scalars):
+        out[i] = int_to_limbs(int(s), limbs)
+    return out
+
+
+# ------------------------------------------------------------ typed points
+
+class _Affine:
+    DTYPE: np.dtype
+    CURVE: int
+
+    def __init__(self, rec: np.ndarray):
+        self.rec = np.ascontiguousarray(rec, dtype=self.DTYPE).reshape(1)
+
+    @classmethod
+    def infinity(cls):
+        r = np.zeros(1, dtype=cls.DTYPE)
+        r["inf"] = 1
+        return cls(r)
+
+    def is_infinity(self) -> bool:
+        fn = "g1_is_infinity_wrapper" if self.CURVE == CURVE_G1 else "g2_is_infinity_wrapper"
+        return bool(getattr(lib(), fn)(_ptr(self.rec)))
+
+    def __eq__(self, other) -> bool:
+        return isinstance(other, type(self)) and self.rec.tobytes() == other.rec.tobytes()
+
+    def __repr__(self) -> str:
+        return f"{type(self).__name__}({'inf' if self.rec['inf'][0] else hex(limbs_to_int(self.rec['x'][0]))})"
+
+
+class _Projective:
+    DTYPE: np.dtype
+    AFF: type
+    CURVE: int
+
+    def __init__(self, rec: np.ndarray):
+        self.rec = np.ascontiguousarray(rec, dtype=self.DTYPE).reshape(1)
+
+    @classmethod
+    def infinity(cls):
+        return cls(np.zeros(1, dtype=cls.DTYPE))
+
+    def to_affine(self):
+        """reference types.rs:331-341 -> projective_to_affine_*_wrapper."""
+        out = np.zeros(1, dtype=self.AFF.DTYPE)
+        fn = "projective_to_affine_g1_wrapper" if self.CURVE == CURVE_G1 else "projective_to_affine_g2_wrapper"
+        getattr(lib(), fn)(_ptr(out), _ptr(self.rec))
+        return self.AFF(out)
+
+    @classmethod
+    def msm(cls, points, scalars, gpu_index: int = 0):
+        """sum_i scalars[i] * points[i] with one u64 scalar per point
+        (reference types.rs:345-376 / 492-532): G1 points in standard form,
+        G2 points in Montgomery form; empty input gives infinity."""
+        pts = _points_array(points, cls.AFF.DTYPE)
+        sc = np.ascontiguousarray(np.asarray(scalars, dtype=np.uint64))
+        if len(pts) != len(sc):
+            raise MsmError(f"Points and scalars must have the same length: {len(pts)} != {len(sc)}")
+        if len(pts) == 0:
+            return cls.infinity()
+        out = np.zeros(1, dtype=cls.DTYPE)
+        fn = "g1_msm_wrapper" if cls.CURVE == CURVE_G1 else "g2_msm_wrapper"
+        _check(getattr(lib(), fn)(_ptr(out), _ptr(pts), _ptr(sc), len(pts), gpu_index), "MSM computation")
+        return cls(out)
+
+    @classmethod
+    def msm_multi(cls, points, scalars: np.ndarray, gpu_index: int = 0, montgomery: bool = False,
+                  scalars_lt_r: bool = False):
+        """Multi-limb scalars shaped (n, limbs) (reference point_msm<T>,
+        src/curve.cu:2763-2826) with an explicit input form."""
+        pts = _points_array(points, cls.AFF.DTYPE)
+        sc = np.ascontiguousarray(np.asarray(scalars, dtype=np.uint64))
+        if sc.ndim == 1:
+            sc = sc.reshape(-1, 1)
+        if len(pts) != len(sc):
+            raise MsmError(f"Points and scalars must have the same length: {len(pts)} != {len(sc)}")
+        if len(pts) == 0:
+            return cls.infinity()
+        flags = (POINTS_MONTGOMERY if montgomery else 0) | (SCALARS_LT_R if scalars_lt_r else 0)
+        out = np.zeros(1, dtype=cls.DTYPE)
+        fn = "g1_msm_multi_wrapper" if cls.CURVE == CURVE_G1 else "g2_msm_multi_wrapper"
+        _check(getattr(lib(), fn)(_ptr(out), _ptr(pts), _ptr(sc), sc.shape[1], len(pts), gpu_index, flags),
+               "MSM computation")
+        return cls(out)
+
+
+def _points_array(points, dtype) -> np.ndarray:
+    if isinstance(points, np.ndarray):
+        return np.ascontiguousarray(points, dtype=dtype)
+    return np.ascontiguousarray(np.concatenate([p.rec for p in points]) if len(points) else np.zeros(0, dtype))
+
+
+class G1Affine(_Affine):
+    DTYPE = G1_DTYPE
+    CURVE = CURVE_G1
+
+
+class G2Affine(_Affine):
+    DTYPE = G2_DTYPE
+    CURVE = CURVE_G2
+
+
+class G1Projective(_Projective):
+    DTYPE = G1P_DTYPE
+    AFF = G1Affine
+    CURVE = CURVE_G1
+
+
+class G2Projective(_Projective):
+    DTYPE = G2P_DTYPE
+    AFF = G2Affine
+    CURVE = CURVE_G2
+
+
+def from_montgomery(curve: int, pts: np.ndarray) -> np.ndarray:
+    """reference c_wrapper.cu:261-283, elementwise over an array of records."""
+    out = np.zeros_like(pts)
+    fn = "g1_from_montgomery_wrapper" if curve == CURVE_G1 else "g2_from_montgomery_wrapper"
+    f = getattr(lib(), fn)
+    size = pts.dtype.itemsize
+    for k in range(len(pts)):
+        f(out.ctypes.data + k * size, pts.ctypes.data + k * size)
+    return out
+
+
+# ---------------------------------------------------------- resident data
+
+class Bases:
+    """Affine bases resident in HBM in the engine's format."""
+
+    def __init__(self, handle: int, curve: int, n: int):
+        self.h, self.curve, self.n = handle, curve, n
+
+    @classmethod
+    def upload(cls, curve: int, points: np.ndarray, montgomery: bool = True, gpu_index: int = 0) -> "Bases":
+        pts = np.ascontiguousarray(points, dtype=G1_DTYPE if curve == CURVE_G1 else G2_DTYPE)
+        h = ctypes.c_void_p()
+        _check(lib().gm_bases_upload(ctypes.byref(h), curve, _ptr(pts), len(pts), int(montgomery), gpu_index),
+               "bases upload")
+        return cls(h.value, curve, len(pts))
+
+    @classmethod
+    def from_k(cls, curve: int, k: np.ndarray, gpu_index: int = 0) -> "Bases":
+        """P_i = k_i * G computed on the device."""
+        k = np.ascontiguousarray(k, dtype=np.uint64)
+        h = ctypes.c_void_p()
+        _check(lib().gm_bases_from_k(ctypes.byref(h), curve, _ptr(k), len(k), gpu_index), "bases generation")
+        return cls(h.value, curve, len(k))
+
+    def download(self, offset: int = 0, n: int | None = None) -> np.ndarray:
+        n = self.n - offset if n is None else n
+        out = np.zeros(n, dtype=G1_DTYPE if self.curve == CURVE_G1 else G2_DTYPE)
+        _check(lib().gm_bases_download(self.h, _ptr(out), offset, n), "bases download")
+        return out
+
+    def free(self) -> None:
+        if self.h:
+            lib().gm_bases_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Scalars:
+    """Scalars resident in HBM, (n, limbs) little-endian u64."""
+
+    def __init__(self, handle: int, n: int, limbs: int):
+        self.h, self.n, self.limbs = handle, n, limbs
+
+    @classmethod
+    def upload(cls, scalars: np.ndarray, gpu_index: int = 0) -> "Scalars":
+        sc = np.ascontiguousarray(scalars, dtype=np.uint64)
+        if sc.ndim == 1:
+            sc = sc.reshape(-1, 1)
+        h = ctypes.c_void_p()
+        _check(lib().gm_scalars_upload(ctypes.byref(h), _ptr(sc), sc.shape[0], sc.shape[1], gpu_index),
+               "scalars upload")
+        return cls(h.value, sc.shape[0], sc.shape[1])
+
+    def free(self) -> None:
+        if self.h:
+            lib().gm_scalars_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def msm_resident(bases: Bases, scalars: Scalars, n: int | None = None, base_offset: int = 0, scalar_offset: int = 0,
+                 scalar_bits: int = 0, window: int = 0, device_combine: bool = False) -> np.ndarray:
+    """MSM over resident data; returns one affine ABI record (Montgomery form)."""
+    n = min(bases.n - base_offset, scalars.n - scalar_offset) if n is None else n
+    out = np.zeros(1, dtype=G1_DTYPE if bases.curve == CURVE_G1 else G2_DTYPE)
+    _check(lib().gm_msm_resident(_ptr(out), bases.h, base_offset, scalars.h, scalar_offset, n, scalar_bits, window,
+                                 int(device_combine)), "resident MSM")
+    return out
+
+
+def plan_query(curve: int, n: int, limbs: int, bits: int = 0, window: int = 0):
+    c, nw, ws = ctypes.c_int(), ctypes.c_int(), ctypes.c_size_t()
+    _check(lib().gm_plan_query(curve, n, limbs, bits, window, ctypes.byref(c), ctypes.byref(nw), ctypes.byref(ws)),
+           "plan query")
+    return c.value, nw.value, ws.value
+
+
+def set_profiling(on: bool, gpu_index: int = 0) -> None:
+    _check(lib().gm_set_profiling(gpu_index, int(on)), "set profiling")
+
+
+STAGES = ["hist", "scan", "scatter", "accumulate", "bucket_reduce", "window_sum"]
+
+
+def profile_last(gpu_index: int = 0) -> dict:
+    buf = (ctypes.c_float * len(STAGES))()
+    _check(lib().gm_profile_last(gpu_index, buf, len(STAGES)), "profile read")
+    return dict(zip(STAGES, list(buf)))

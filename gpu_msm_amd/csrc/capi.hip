@@ -1,0 +1,737 @@
+// C ABI of the MSM library: the reference's rust_wrapper.h surface
+// (reference rust-api/src/c_wrapper.cu) plus the extensions of
+// include/gpu_msm_ext.h. Host code: per-device context (stream, cached
+// workspace, pinned staging), plan selection, the host window combine.
+#include <hipcub/hipcub.hpp>
+
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <mutex>
+
+#include "../../include/gpu_msm_ext.h"
+#include "../../include/rust_wrapper.h"
+#include "engine.hpp"
+#include "host_field.hpp"
+
+using gm::Aff;
+using gm::Fp2Ops;
+using gm::FpOps;
+using gm::Plan;
+using gm::Xyzz;
+
+namespace gm {
+
+size_t xyzz_bytes(int curve) { return curve == CURVE_G1 ? sizeof(Xyzz<FpOps>) : sizeof(Xyzz<Fp2Ops>); }
+size_t aff_bytes(int curve) { return curve == CURVE_G1 ? sizeof(Aff<FpOps>) : sizeof(Aff<Fp2Ops>); }
+
+static int windows_for(int bits, int c) { return (bits + 1 + c - 1) / c; }
+
+static int seg_for(int B) { return B < 32 ? B : 32; }
+
+// Relative cost in Fp-multiplication equivalents: mixed adds of the bucket
+// phase, running-sum adds and the per-segment scalar multiples of the bucket
+// reduction, and a per-(window, point) charge for recoding and sorting.
+static double plan_cost(int curve, int n, int bits, int c) {
+    const int nw = windows_for(bits, c);
+    const double B = (double)(1 << (c - 1));
+    const double seg = seg_for(1 << (c - 1));
+    const double fm = curve == CURVE_G1 ? 1.0 : 3.0;
+    const double accum = (double)nw * n * 10.5 * fm;
+    const double reduce = (double)nw * (B * 28.0 + (B / seg) * 18.0 * c) * fm;
+    const double sort = (double)nw * n * 0.6;
+    return accum + reduce + sort;
+}
+
+int choose_window(int curve, int n, int bits) {
+    int best = 2;
+    double bc = 1e300;
+    for (int c = 2; c <= 20; ++c) {
+        const double v = plan_cost(curve, n, bits, c);
+        if (v < bc) {
+            bc = v;
+            best = c;
+        }
+    }
+    return best;
+}
+
+size_t al_size(size_t x) { return (x + 255) & ~(size_t)255; }
+static size_t al(size_t x) { return al_size(x); }
+
+bool make_plan(Plan& p, int curve, int n, int L, int bits, int c) {
+    memset(&p, 0, sizeof p);
+    if (n < 0 || L < 1 || L > 8) return false;
+    if (bits <= 0 || bits > 64 * L) bits = 64 * L;
+    if (c <= 0) c = choose_window(curve, n, bits);
+    if (c < 2 || c > 24) return false;
+    p.curve = curve;
+    p.n = n;
+    p.L = L;
+    p.bits = bits;
+    p.c = c;
+    p.nw = windows_for(bits, c);
+    p.B = 1 << (c - 1);
+    p.seg = seg_for(p.B);
+    p.nseg = p.B / p.seg;
+    const size_t nbk = (size_t)p.nw * p.B;
+    size_t cub = 0;
+    if (hipcub::DeviceScan::ExclusiveSum(nullptr, cub, (uint32_t*)nullptr, (uint32_t*)nullptr, (int)(nbk + 1)) !=
+        hipSuccess) {
+        // No device to ask (plan queries on a host without a GPU): a bound on
+        // the decoupled-lookback scan state.
+        cub = (nbk / 128 + 1024) * 16 + 65536;
+    }
+    p.cub_bytes = cub;
+    const size_t xb = xyzz_bytes(curve);
+    size_t o = 0;
+    p.o_cnt = o;
+    o += al((nbk + 1) * 4);
+    p.o_off = o;
+    o += al((nbk + 1) * 4);
+    p.o_rank = o;
+    o += al((size_t)p.nw * n * 4 + 4);
+    p.o_sorted = o;
+    o += al((size_t)p.nw * n * 4 + 4);
+    p.o_buckets = o;
+    o += al(nbk * xb);
+    p.o_segs = o;
+    o += al((size_t)p.nw * p.nseg * xb);
+    p.o_wsum = o;
+    o += al((size_t)p.nw * xb);
+    p.o_out = o;
+    o += al(aff_bytes(curve));
+    p.o_err = o;
+    o += al(4);
+    p.o_cub = o;
+    o += al(cub);
+    p.total = o;
+    return true;
+}
+
+}  // namespace gm
+
+// ------------------------------------------------------------------ context
+
+namespace {
+
+struct DevCtx {
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    uint8_t* ws = nullptr;
+    size_t cap = 0;
+    uint8_t* pinned = nullptr;  // host staging for window sums / results
+    size_t pinned_cap = 0;
+    std::mutex mu;
+    gm::Prof prof{};
+    bool prof_valid = false;
+};
+
+std::mutex g_ctx_mu;
+DevCtx* g_ctx[64];
+
+DevCtx* get_ctx(uint32_t gpu) {
+    if (gpu >= 64) return nullptr;
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    if (g_ctx[gpu]) return g_ctx[gpu];
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || (int)gpu >= count) return nullptr;
+    if (hipSetDevice((int)gpu) != hipSuccess) return nullptr;
+    DevCtx* c = new DevCtx();
+    c->dev = (int)gpu;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return nullptr;
+    }
+    for (int i = 0; i < gm::ST_COUNT; ++i)
+        if (hipEventCreate(&c->prof.ev[i]) != hipSuccess) {
+            delete c;
+            return nullptr;
+        }
+    c->prof.on = false;
+    g_ctx[gpu] = c;
+    return c;
+}
+
+hipError_t ensure_ws(DevCtx* c, size_t bytes) {
+    if (bytes <= c->cap) return hipSuccess;
+    if (c->ws) {
+        hipStreamSynchronize(c->stream);
+        hipFree(c->ws);
+        c->ws = nullptr;
+        c->cap = 0;
+    }
+    size_t want = bytes + bytes / 8;
+    hipError_t e = hipMalloc(&c->ws, want);
+    if (e != hipSuccess) {
+        c->ws = nullptr;
+        return e;
+    }
+    c->cap = want;
+    return hipSuccess;
+}
+
+hipError_t ensure_pinned(DevCtx* c, size_t bytes) {
+    if (bytes <= c->pinned_cap) return hipSuccess;
+    if (c->pinned) {
+        hipStreamSynchronize(c->stream);
+        hipHostFree(c->pinned);
+        c->pinned = nullptr;
+        c->pinned_cap = 0;
+    }
+    hipError_t e = hipHostMalloc((void**)&c->pinned, bytes, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        c->pinned = nullptr;
+        return e;
+    }
+    c->pinned_cap = bytes;
+    return hipSuccess;
+}
+
+// ----------------------------------------------------- host-side conversions
+
+void dev_elem_to_host(gmh::Fp& r, const uint32_t* v) {
+    gmh::Fp t;
+    gmh::from_limbs28(t, v);
+    gmh::reduce(r, t);
+}
+void dev_elem_to_host(gmh::Fp2& r, const uint32_t* v) {
+    dev_elem_to_host(r.c0, v);
+    dev_elem_to_host(r.c1, v + 16);
+}
+
+template <class HF>
+constexpr int words_of() {
+    return sizeof(HF) == sizeof(gmh::Fp) ? 16 : 32;
+}
+
+// Horner over nw XYZZ window sums (raw device words), most significant first.
+template <class HF>
+bool host_combine(HF& x, HF& y, const uint32_t* raw, int nw, int c) {
+    constexpr int W = words_of<HF>();
+    auto load = [&](gmh::Xyzz<HF>& p, int w) {
+        const uint32_t* b = raw + (size_t)w * 4 * W;
+        dev_elem_to_host(p.X, b);
+        dev_elem_to_host(p.Y, b + W);
+        dev_elem_to_host(p.ZZ, b + 2 * W);
+        dev_elem_to_host(p.ZZZ, b + 3 * W);
+    };
+    gmh::Xyzz<HF> r, v;
+    load(r, nw - 1);
+    for (int w = nw - 2; w >= 0; --w) {
+        for (int k = 0; k < c; ++k) gmh::xdbl(r, r);
+        load(v, w);
+        gmh::xadd(r, r, v);
+    }
+    return gmh::to_affine(x, y, r);
+}
+
+void put_elem(uint64_t* w, const gmh::Fp& a) { memcpy(w, a.l, 56); }
+void put_elem(uint64_t* w, const gmh::Fp2& a) {
+    memcpy(w, a.c0.l, 56);
+    memcpy(w + 7, a.c1.l, 56);
+}
+void get_elem(gmh::Fp& a, const uint64_t* w) { memcpy(a.l, w, 56); }
+void get_elem(gmh::Fp2& a, const uint64_t* w) {
+    memcpy(a.c0.l, w, 56);
+    memcpy(a.c1.l, w + 7, 56);
+}
+
+// Writes an affine result as an ABI affine record or an ABI projective point.
+template <class HF>
+void write_affine_abi(void* out, bool finite, const HF& x, const HF& y) {
+    constexpr int E = sizeof(HF) / 8;
+    uint64_t buf[2 * E + 1];
+    memset(buf, 0, sizeof buf);
+    if (finite) {
+        put_elem(buf, x);
+        put_elem(buf + E, y);
+    } else {
+        buf[2 * E] = 1;
+    }
+    memcpy(out, buf, sizeof buf);
+}
+template <class HF>
+void write_proj_abi(void* out, bool finite, const HF& x, const HF& y) {
+    constexpr int E = sizeof(HF) / 8;
+    uint64_t buf[3 * E];
+    memset(buf, 0, sizeof buf);
+    if (finite) {
+        put_elem(buf, x);
+        put_elem(buf + E, y);
+        buf[2 * E + 0] = 0;
+        memcpy(buf + 2 * E, gmh::ONE.l, 56);
+    }
+    memcpy(out, buf, sizeof buf);
+}
+
+template <class O>
+struct HostOf;
+template <>
+struct HostOf<FpOps> {
+    using HF = gmh::Fp;
+    static constexpr int CURVE = gm::CURVE_G1;
+};
+template <>
+struct HostOf<Fp2Ops> {
+    using HF = gmh::Fp2;
+    static constexpr int CURVE = gm::CURVE_G2;
+};
+
+// Runs the device pipeline for an already-resident set of bases/scalars and
+// finishes on the host (or device). Caller holds c->mu. Writes affine x, y.
+template <class O>
+int run_msm_locked(DevCtx* c, const Aff<O>* d_bases, const uint64_t* d_scalars, int n, int L, int bits, int window,
+                   bool device_combine, typename HostOf<O>::HF& x, typename HostOf<O>::HF& y, bool& finite,
+                   uint8_t* ws_override = nullptr) {
+    using HF = typename HostOf<O>::HF;
+    Plan p;
+    if (!gm::make_plan(p, HostOf<O>::CURVE, n, L, bits, window)) return -1;
+    uint8_t* ws = ws_override;
+    if (!ws) {
+        if (ensure_ws(c, p.total) != hipSuccess) return -3;
+        ws = c->ws;
+    }
+    const size_t wsb = (size_t)p.nw * sizeof(Xyzz<O>);
+    if (ensure_pinned(c, wsb + sizeof(Aff<O>) + 256) != hipSuccess) return -3;
+    if (gm::Engine<O>::run(c->stream, p, d_bases, d_scalars, ws, &c->prof) != hipSuccess) return -4;
+    c->prof_valid = c->prof.on;
+    uint8_t* h = c->pinned;
+    if (hipMemcpyAsync(h + wsb + sizeof(Aff<O>), ws + p.o_err, 4, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+        return -4;
+    if (device_combine) {
+        if (gm::Engine<O>::device_combine(c->stream, p, ws) != hipSuccess) return -4;
+        if (hipMemcpyAsync(h + wsb, ws + p.o_out, sizeof(Aff<O>), hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+            return -4;
+    } else {
+        if (hipMemcpyAsync(h, ws + p.o_wsum, wsb, hipMemcpyDeviceToHost, c->stream) != hipSuccess) return -4;
+    }
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return -4;
+    uint32_t err;
+    memcpy(&err, h + wsb + sizeof(Aff<O>), 4);
+    if (err) return -5;
+    if (device_combine) {
+        const uint32_t* a = reinterpret_cast<const uint32_t*>(h + wsb);
+        constexpr int W = words_of<HF>();
+        finite = a[15] != gm::INF_MARK;
+        if (finite) {
+            dev_elem_to_host(x, a);
+            dev_elem_to_host(y, a + W);
+        }
+    } else {
+        finite = host_combine<HF>(x, y, reinterpret_cast<const uint32_t*>(h), p.nw, p.c);
+    }
+    return 0;
+}
+
+// Host-pointer MSM: upload, convert, run, finish. Used by the reference-ABI
+// wrappers.
+template <class O>
+int msm_host_ptrs(void* result_proj, const void* points, const uint64_t* scalars, int L, int bits, int n,
+                  uint32_t gpu, bool to_mont) {
+    using HF = typename HostOf<O>::HF;
+    using Lay = gm::AbiLayout<O>;
+    if (n <= 0 || !result_proj || !points || !scalars || L < 1 || L > 8) return -1;
+    DevCtx* c = get_ctx(gpu);
+    if (!c) return -2;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (hipSetDevice(c->dev) != hipSuccess) return -2;
+    Plan p;
+    if (!gm::make_plan(p, HostOf<O>::CURVE, n, L, bits, 0)) return -1;
+    const size_t b_abi = gm::al_size((size_t)n * Lay::AFF_BYTES);
+    const size_t b_aff = gm::al_size((size_t)n * sizeof(Aff<O>));
+    const size_t b_sc = gm::al_size((size_t)n * L * 8);
+    if (ensure_ws(c, b_abi + b_aff + b_sc + p.total) != hipSuccess) return -3;
+    uint8_t* d_abi = c->ws;
+    Aff<O>* d_aff = reinterpret_cast<Aff<O>*>(c->ws + b_abi);
+    uint64_t* d_sc = reinterpret_cast<uint64_t*>(c->ws + b_abi + b_aff);
+    uint8_t* ws = c->ws + b_abi + b_aff + b_sc;
+    if (hipMemcpyAsync(d_abi, points, (size_t)n * Lay::AFF_BYTES, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        return -4;
+    if (hipMemcpyAsync(d_sc, scalars, (size_t)n * L * 8, hipMemcpyHostToDevice, c->stream) != hipSuccess) return -4;
+    if (gm::Engine<O>::import_aff(c->stream, d_aff, d_abi, n, to_mont) != hipSuccess) return -4;
+    HF x, y;
+    bool finite = false;
+    int rc = run_msm_locked<O>(c, d_aff, d_sc, n, L, bits, 0, false, x, y, finite, ws);
+    if (rc) return rc;
+    write_proj_abi<HF>(result_proj, finite, x, y);
+    return 0;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ resident data
+
+struct gm_bases {
+    int curve;
+    int n;
+    int dev;
+    void* d;  // Aff<O>[n]
+};
+
+struct gm_scalars {
+    int n;
+    int limbs;
+    int dev;
+    uint64_t* d;
+};
+
+namespace {
+
+template <class O>
+Aff<O> generator_dev();
+template <>
+Aff<FpOps> generator_dev<FpOps>() {
+    // reference src/curve.cu:786-796 (standard form)
+    static const gmh::Fp gx = {{0x3bf9166c8236f4faULL, 0x8bc02b7cbe6a9e8dULL, 0x11c1e56b3e4bc80bULL,
+                                0x6b20d782901a6f62ULL, 0x2ce8c34265bf3841ULL, 0x11b73d3d76ae9851ULL,
+                                0x326ed6bd777fc6a3ULL}};
+    static const gmh::Fp gy = {{0xfe6f792612016b30ULL, 0x22db0ce6034a9db9ULL, 0xb9093f32002756daULL,
+                                0x39d7f424b6660204ULL, 0xf843c947aa57f571ULL, 0xd6d62d244e413636ULL,
+                                0x1a7caf4a4d3887a6ULL}};
+    Aff<FpOps> g;
+    gmh::Fp t;
+    gmh::to_mont(t, gx);
+    gmh::to_limbs28(g.x.v, t);
+    gmh::to_mont(t, gy);
+    gmh::to_limbs28(g.y.v, t);
+    return g;
+}
+template <>
+Aff<Fp2Ops> generator_dev<Fp2Ops>() {
+    // reference src/curve.cu:798-814 (standard form)
+    static const gmh::Fp c[4] = {
+        {{0x0e529ee4dce9991dULL, 0xd6ebaf149094f1ccULL, 0x043c6bf16312d638ULL, 0x062b61439640e885ULL,
+          0x18dad8ed784dd225ULL, 0xa57c0038441f7d15ULL, 0x21f8d4a76f74541aULL}},
+        {{0xcaf5185423a7d23aULL, 0x7cef6acb145b6413ULL, 0x2879dd439b019b8bULL, 0x71449cdeca4f0007ULL,
+          0xdebaf4a2c5534527ULL, 0xa1b4e791d1b86560ULL, 0x1e0f563c601bb8dcULL}},
+        {{0x274315837455b919ULL, 0x82039e4221ff3507ULL, 0x00346cebad16a036ULL, 0x0177bfd6654e681eULL,
+          0xddff621b5db3f897ULL, 0x0cc61570301497a7ULL, 0x115ea2305a78f646ULL}},
+        {{0x392236e9cf2976c2ULL, 0xd8ab17c84b9f03cdULL, 0x8a8e6755f9d82fd1ULL, 0x7532834528cd5a64ULL,
+          0x0b0bcc3fb6f2161cULL, 0x76a2ffcb7d47679dULL, 0x25ed2192b203c1feULL}}};
+    Aff<Fp2Ops> g;
+    gmh::Fp t;
+    gmh::to_mont(t, c[0]);
+    gmh::to_limbs28(g.x.c0.v, t);
+    gmh::to_mont(t, c[1]);
+    gmh::to_limbs28(g.x.c1.v, t);
+    gmh::to_mont(t, c[2]);
+    gmh::to_limbs28(g.y.c0.v, t);
+    gmh::to_mont(t, c[3]);
+    gmh::to_limbs28(g.y.c1.v, t);
+    return g;
+}
+
+template <class O>
+int bases_create(gm_bases** out, const void* host_points, const uint64_t* host_k, int n, int montgomery,
+                 uint32_t gpu) {
+    using Lay = gm::AbiLayout<O>;
+    if (!out || n <= 0) return -1;
+    *out = nullptr;
+    DevCtx* c = get_ctx(gpu);
+    if (!c) return -2;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (hipSetDevice(c->dev) != hipSuccess) return -2;
+    void* d = nullptr;
+    if (hipMalloc(&d, (size_t)n * sizeof(Aff<O>)) != hipSuccess) return -3;
+    void* tmp = nullptr;
+    const size_t tbytes = host_points ? (size_t)n * Lay::AFF_BYTES : (size_t)n * 8;
+    if (hipMalloc(&tmp, tbytes) != hipSuccess) {
+        hipFree(d);
+        return -3;
+    }
+    hipError_t e = hipMemcpyAsync(tmp, host_points ? host_points : (const void*)host_k, tbytes,
+                                  hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        if (host_points)
+            e = gm::Engine<O>::import_aff(c->stream, reinterpret_cast<Aff<O>*>(d), tmp, n, montgomery == 0);
+        else
+            e = gm::Engine<O>::gen_bases(c->stream, reinterpret_cast<Aff<O>*>(d), reinterpret_cast<uint64_t*>(tmp), n,
+                                         generator_dev<O>());
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    hipFree(tmp);
+    if (e != hipSuccess) {
+        hipFree(d);
+        return -4;
+    }
+    gm_bases* b = new gm_bases{HostOf<O>::CURVE, n, c->dev, d};
+    *out = b;
+    return 0;
+}
+
+template <class O>
+int resident_msm(void* result_affine, const gm_bases* b, int base_offset, const gm_scalars* s, int scalar_offset,
+                 int n, int bits, int window, int device_combine) {
+    using HF = typename HostOf<O>::HF;
+    if (!result_affine || !b || !s || n <= 0 || base_offset < 0 || scalar_offset < 0) return -1;
+    if ((long)base_offset + n > b->n || (long)scalar_offset + n > s->n || b->dev != s->dev) return -1;
+    DevCtx* c = get_ctx((uint32_t)b->dev);
+    if (!c) return -2;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (hipSetDevice(c->dev) != hipSuccess) return -2;
+    HF x, y;
+    bool finite = false;
+    const Aff<O>* bases = reinterpret_cast<const Aff<O>*>(b->d) + base_offset;
+    const uint64_t* sc = s->d + (size_t)scalar_offset * s->limbs;
+    int rc = run_msm_locked<O>(c, bases, sc, n, s->limbs, bits, window, device_combine != 0, x, y, finite);
+    if (rc) return rc;
+    write_affine_abi<HF>(result_affine, finite, x, y);
+    return 0;
+}
+
+}  // namespace
+
+// ================================================================= C ABI
+
+extern "C" {
+
+// reference rust-api/src/c_wrapper.cu:12-19
+void affine_to_projective_g1_wrapper(G1ProjectivePoint* proj, const G1Point* affine) {
+    memset(proj, 0, sizeof *proj);
+    if (affine->infinity) return;
+    proj->X = affine->x;
+    proj->Y = affine->y;
+    memcpy(proj->Z.limb, gmh::ONE.l, 56);
+}
+void affine_to_projective_g2_wrapper(G2ProjectivePoint* proj, const G2Point* affine) {
+    memset(proj, 0, sizeof *proj);
+    if (affine->infinity) return;
+    proj->X = affine->x;
+    proj->Y = affine->y;
+    memcpy(proj->Z.c0.limb, gmh::ONE.l, 56);
+}
+
+// reference rust-api/src/c_wrapper.cu:22-29 (src/curve.cu:754-778)
+void projective_to_affine_g1_wrapper(G1Point* affine, const G1ProjectivePoint* proj) {
+    gmh::Fp X, Y, Z, zi, x, y;
+    get_elem(X, proj->X.limb);
+    get_elem(Y, proj->Y.limb);
+    get_elem(Z, proj->Z.limb);
+    G1Point r;
+    memset(&r, 0, sizeof r);
+    if (gmh::is_zero(Z)) {
+        r.infinity = true;
+    } else {
+        gmh::inv(zi, Z);
+        gmh::mul(x, X, zi);
+        gmh::mul(y, Y, zi);
+        memcpy(r.x.limb, x.l, 56);
+        memcpy(r.y.limb, y.l, 56);
+    }
+    *affine = r;
+}
+void projective_to_affine_g2_wrapper(G2Point* affine, const G2ProjectivePoint* proj) {
+    gmh::Fp2 X, Y, Z, zi, x, y;
+    get_elem(X, proj->X.c0.limb);
+    get_elem(Y, proj->Y.c0.limb);
+    get_elem(Z, proj->Z.c0.limb);
+    G2Point r;
+    memset(&r, 0, sizeof r);
+    if (gmh::is_zero(Z)) {
+        r.infinity = true;
+    } else {
+        gmh::inv(zi, Z);
+        gmh::mul(x, X, zi);
+        gmh::mul(y, Y, zi);
+        put_elem(r.x.c0.limb, x);
+        put_elem(r.y.c0.limb, y);
+    }
+    *affine = r;
+}
+
+// reference rust-api/src/c_wrapper.cu:32-59
+void g1_point_at_infinity_wrapper(G1Point* point) {
+    memset(point, 0, sizeof *point);
+    point->infinity = true;
+}
+void g2_point_at_infinity_wrapper(G2Point* point) {
+    memset(point, 0, sizeof *point);
+    point->infinity = true;
+}
+void g1_projective_point_at_infinity_wrapper(G1ProjectivePoint* point) { memset(point, 0, sizeof *point); }
+void g2_projective_point_at_infinity_wrapper(G2ProjectivePoint* point) { memset(point, 0, sizeof *point); }
+bool g1_is_infinity_wrapper(const G1Point* point) { return point->infinity; }
+bool g2_is_infinity_wrapper(const G2Point* point) { return point->infinity; }
+
+// reference rust-api/src/c_wrapper.cu:64-172
+int g1_msm_wrapper(G1ProjectivePoint* result, const G1Point* points, const uint64_t* scalars, int n,
+                   uint32_t gpu_index) {
+    return msm_host_ptrs<FpOps>(result, points, scalars, 1, 64, n, gpu_index, true);
+}
+
+// reference rust-api/src/c_wrapper.cu:175-258
+int g2_msm_wrapper(G2ProjectivePoint* result, const G2Point* points, const uint64_t* scalars, int n,
+                   uint32_t gpu_index) {
+    return msm_host_ptrs<Fp2Ops>(result, points, scalars, 1, 64, n, gpu_index, false);
+}
+
+// reference rust-api/src/c_wrapper.cu:261-283
+void g1_from_montgomery_wrapper(G1Point* result, const G1Point* point) {
+    G1Point r;
+    memset(&r, 0, sizeof r);
+    if (point->infinity) {
+        r.infinity = true;
+    } else {
+        gmh::Fp a, b;
+        get_elem(a, point->x.limb);
+        gmh::from_mont(b, a);
+        memcpy(r.x.limb, b.l, 56);
+        get_elem(a, point->y.limb);
+        gmh::from_mont(b, a);
+        memcpy(r.y.limb, b.l, 56);
+    }
+    *result = r;
+}
+void g2_from_montgomery_wrapper(G2Point* result, const G2Point* point) {
+    G2Point r;
+    memset(&r, 0, sizeof r);
+    if (point->infinity) {
+        r.infinity = true;
+    } else {
+        const Fp* in[4] = {&point->x.c0, &point->x.c1, &point->y.c0, &point->y.c1};
+        Fp* out[4] = {&r.x.c0, &r.x.c1, &r.y.c0, &r.y.c1};
+        for (int k = 0; k < 4; ++k) {
+            gmh::Fp a, b;
+            get_elem(a, in[k]->limb);
+            gmh::from_mont(b, a);
+            memcpy(out[k]->limb, b.l, 56);
+        }
+    }
+    *result = r;
+}
+// reference rust-api/src/c_wrapper.cu:286-288
+void fp_to_montgomery_wrapper(Fp* result, const Fp* value) {
+    gmh::Fp a, b;
+    get_elem(a, value->limb);
+    gmh::to_mont(b, a);
+    memcpy(result->limb, b.l, 56);
+}
+
+// ------------------------------------------------------------- extensions
+
+int g1_msm_multi_wrapper(G1ProjectivePoint* result, const G1Point* points, const uint64_t* scalars,
+                         int scalar_limbs, int n, uint32_t gpu_index, uint32_t flags) {
+    const int bits = (flags & GM_SCALARS_LT_R) ? 299 : 64 * scalar_limbs;
+    return msm_host_ptrs<FpOps>(result, points, scalars, scalar_limbs, bits, n, gpu_index,
+                                (flags & GM_POINTS_MONTGOMERY) == 0);
+}
+int g2_msm_multi_wrapper(G2ProjectivePoint* result, const G2Point* points, const uint64_t* scalars,
+                         int scalar_limbs, int n, uint32_t gpu_index, uint32_t flags) {
+    const int bits = (flags & GM_SCALARS_LT_R) ? 299 : 64 * scalar_limbs;
+    return msm_host_ptrs<Fp2Ops>(result, points, scalars, scalar_limbs, bits, n, gpu_index,
+                                 (flags & GM_POINTS_MONTGOMERY) == 0);
+}
+
+int gm_plan_query(int curve, int n, int scalar_limbs, int scalar_bits, int window, int* c_out, int* windows_out,
+                  size_t* workspace_bytes_out) {
+    if (curve != GM_CURVE_G1 && curve != GM_CURVE_G2) return -1;
+    Plan p;
+    if (!gm::make_plan(p, curve, n, scalar_limbs, scalar_bits, window)) return -1;
+    if (c_out) *c_out = p.c;
+    if (windows_out) *windows_out = p.nw;
+    if (workspace_bytes_out) *workspace_bytes_out = p.total;
+    return 0;
+}
+
+int gm_bases_upload(gm_bases** out, int curve, const void* host_points, int n, int montgomery, uint32_t gpu_index) {
+    if (!host_points) return -1;
+    if (curve == GM_CURVE_G1) return bases_create<FpOps>(out, host_points, nullptr, n, montgomery, gpu_index);
+    if (curve == GM_CURVE_G2) return bases_create<Fp2Ops>(out, host_points, nullptr, n, montgomery, gpu_index);
+    return -1;
+}
+
+int gm_bases_from_k(gm_bases** out, int curve, const uint64_t* host_k, int n, uint32_t gpu_index) {
+    if (!host_k) return -1;
+    if (curve == GM_CURVE_G1) return bases_create<FpOps>(out, nullptr, host_k, n, 1, gpu_index);
+    if (curve == GM_CURVE_G2) return bases_create<Fp2Ops>(out, nullptr, host_k, n, 1, gpu_index);
+    return -1;
+}
+
+int gm_bases_download(const gm_bases* b, void* host_points, int offset, int n) {
+    if (!b || !host_points || offset < 0 || n <= 0 || (long)offset + n > b->n) return -1;
+    DevCtx* c = get_ctx((uint32_t)b->dev);
+    if (!c) return -2;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (hipSetDevice(c->dev) != hipSuccess) return -2;
+    const size_t rec = b->curve == GM_CURVE_G1 ? 120 : 232;
+    void* tmp = nullptr;
+    if (hipMalloc(&tmp, (size_t)n * rec) != hipSuccess) return -3;
+    hipError_t e;
+    if (b->curve == GM_CURVE_G1)
+        e = gm::Engine<FpOps>::export_aff(c->stream, tmp, reinterpret_cast<const Aff<FpOps>*>(b->d) + offset, n);
+    else
+        e = gm::Engine<Fp2Ops>::export_aff(c->stream, tmp, reinterpret_cast<const Aff<Fp2Ops>*>(b->d) + offset, n);
+    if (e == hipSuccess) e = hipMemcpyAsync(host_points, tmp, (size_t)n * rec, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    hipFree(tmp);
+    return e == hipSuccess ? 0 : -4;
+}
+
+int gm_bases_count(const gm_bases* b) { return b ? b->n : 0; }
+
+void gm_bases_free(gm_bases* b) {
+    if (!b) return;
+    hipSetDevice(b->dev);
+    hipFree(b->d);
+    delete b;
+}
+
+int gm_scalars_upload(gm_scalars** out, const uint64_t* host, int n, int limbs, uint32_t gpu_index) {
+    if (!out || !host || n <= 0 || limbs < 1 || limbs > 8) return -1;
+    *out = nullptr;
+    DevCtx* c = get_ctx(gpu_index);
+    if (!c) return -2;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (hipSetDevice(c->dev) != hipSuccess) return -2;
+    uint64_t* d = nullptr;
+    if (hipMalloc(&d, (size_t)n * limbs * 8) != hipSuccess) return -3;
+    if (hipMemcpy(d, host, (size_t)n * limbs * 8, hipMemcpyHostToDevice) != hipSuccess) {
+        hipFree(d);
+        return -4;
+    }
+    *out = new gm_scalars{n, limbs, c->dev, d};
+    return 0;
+}
+
+void gm_scalars_free(gm_scalars* s) {
+    if (!s) return;
+    hipSetDevice(s->dev);
+    hipFree(s->d);
+    delete s;
+}
+
+int gm_msm_resident(void* result_affine, const gm_bases* b, int base_offset, const gm_scalars* s, int scalar_offset,
+                    int n, int scalar_bits, int window, int device_combine) {
+    if (!b) return -1;
+    if (b->curve == GM_CURVE_G1)
+        return resident_msm<FpOps>(result_affine, b, base_offset, s, scalar_offset, n, scalar_bits, window,
+                                   device_combine);
+    return resident_msm<Fp2Ops>(result_affine, b, base_offset, s, scalar_offset, n, scalar_bits, window,
+                                device_combine);
+}
+
+int gm_set_profiling(uint32_t gpu_index, int on) {
+    DevCtx* c = get_ctx(gpu_index);
+    if (!c) return -2;
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->prof.on = on != 0;
+    c->prof_valid = false;
+    return 0;
+}
+
+int gm_profile_last(uint32_t gpu_index, float* stage_ms, int count) {
+    DevCtx* c = get_ctx(gpu_index);
+    if (!c) return -2;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (!c->prof_valid || !stage_ms) return -1;
+    for (int i = 0; i < count && i + 1 < gm::ST_COUNT; ++i) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, c->prof.ev[i], c->prof.ev[i + 1]) != hipSuccess) return -4;
+        stage_ms[i] = ms;
+    }
+    return 0;
+}
+
+}  // extern "C"

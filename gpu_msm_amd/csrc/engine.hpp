@@ -1,0 +1,98 @@
+// Host-side interface of the MSM engine: plans, workspace layout and the
+// per-curve kernel drivers (instantiated in kern_g1.hip / kern_g2.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "curve.hpp"
+
+namespace gm {
+
+enum Curve { CURVE_G1 = 1, CURVE_G2 = 2 };
+
+// Reference ABI record sizes (reference include/curve.h:13-43).
+template <class O>
+struct AbiLayout;
+template <>
+struct AbiLayout<FpOps> {
+    static constexpr int AFF_BYTES = 120;   // G1Point
+    static constexpr int PROJ_BYTES = 168;  // G1ProjectivePoint
+    static constexpr int ELEM_U64 = 7;
+};
+template <>
+struct AbiLayout<Fp2Ops> {
+    static constexpr int AFF_BYTES = 232;   // G2Point
+    static constexpr int PROJ_BYTES = 336;  // G2ProjectivePoint
+    static constexpr int ELEM_U64 = 14;
+};
+
+size_t al_size(size_t x);
+
+// Profiling stages recorded with HIP events on the MSM's own stream.
+enum Stage { ST_START = 0, ST_HIST, ST_SCAN, ST_SCATTER, ST_ACCUM, ST_REDUCE, ST_WSUM, ST_COUNT };
+
+struct Prof {
+    hipEvent_t ev[ST_COUNT];
+    bool on;
+};
+
+struct Plan {
+    int curve;
+    int n;        // points
+    int L;        // u64 limbs per scalar
+    int bits;     // scalar bit bound (values must be < 2^bits)
+    int c;        // window bits
+    int nw;       // windows
+    int B;        // buckets per window = 2^(c-1)
+    int seg;      // buckets per bucket-reduce lane
+    int nseg;     // segments per window
+    size_t cub_bytes;
+    // workspace offsets (bytes)
+    size_t o_cnt, o_off, o_rank, o_sorted, o_buckets, o_segs, o_wsum, o_out, o_err, o_cub, total;
+};
+
+size_t xyzz_bytes(int curve);
+size_t aff_bytes(int curve);
+int choose_window(int curve, int n, int bits);
+// c <= 0 picks the analytic heuristic.
+bool make_plan(Plan& p, int curve, int n, int L, int bits, int c);
+
+template <class O>
+struct Engine {
+    // Runs digits -> sort -> accumulate -> reduce -> window sums into the
+    // workspace (window sums at p.o_wsum). Asynchronous on `s`.
+    static hipError_t run(hipStream_t s, const Plan& p, const Aff<O>* d_bases, const uint64_t* d_scalars, uint8_t* d_ws,
+                          Prof* prof);
+    // Horner over window sums on the device, canonical affine at p.o_out.
+    static hipError_t device_combine(hipStream_t s, const Plan& p, uint8_t* d_ws);
+    static hipError_t import_aff(hipStream_t s, Aff<O>* out, const void* d_abi, int n, bool to_mont);
+    static hipError_t export_aff(hipStream_t s, void* d_abi, const Aff<O>* in, int n);
+    static hipError_t gen_bases(hipStream_t s, Aff<O>* out, const uint64_t* d_k, int n, const Aff<O>& g);
+    static hipError_t aff_to_abi_proj(hipStream_t s, void* d_abi_proj, const Aff<O>* in);
+};
+
+extern template struct Engine<FpOps>;
+extern template struct Engine<Fp2Ops>;
+
+// Per-stage launchers, each compiled in its own translation unit so the
+// heavy G2 instantiations build in parallel.
+hipError_t launch_sort(hipStream_t s, const Plan& p, const uint64_t* d_scalars, uint8_t* ws, Prof* prof);
+template <class O>
+hipError_t launch_accumulate(hipStream_t s, const Plan& p, const Aff<O>* d_bases, uint8_t* ws);
+template <class O>
+hipError_t launch_reduce(hipStream_t s, const Plan& p, uint8_t* ws);
+template <class O>
+hipError_t launch_window_sum(hipStream_t s, const Plan& p, uint8_t* ws);
+template <class O>
+hipError_t launch_horner(hipStream_t s, const Xyzz<O>* wsum, int count, int nw, int c, Aff<O>* out);
+template <class O>
+hipError_t launch_import(hipStream_t s, Aff<O>* out, const void* d_abi, int n, bool to_mont);
+template <class O>
+hipError_t launch_export(hipStream_t s, void* d_abi, const Aff<O>* in, int n);
+template <class O>
+hipError_t launch_gen_bases(hipStream_t s, Aff<O>* out, const uint64_t* d_k, int n, const Aff<O>& g);
+template <class O>
+hipError_t launch_aff_to_abi_proj(hipStream_t s, void* d_abi_proj, const Aff<O>* in, int count);
+
+}  // namespace gm

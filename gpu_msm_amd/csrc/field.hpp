@@ -1,0 +1,324 @@
+// Device field arithmetic for BLS12-446 on gfx950.
+//
+// Fp elements are 16 limbs of 28 bits held in 32-bit VGPRs, value in [0, 2p),
+// Montgomery form with R = 2^448 (16 * 28 = 448, the same R as the reference's
+// 7 x 64-bit representation, reference src/fp.cu:15-33, so the Montgomery
+// images are identical and conversion is a pure bit repacking).
+//
+// Multiplication is product scanning with the Montgomery reduction folded into
+// each column (FIPS). Every limb product is one v_mad_u64_u32 accumulating into
+// a 64-bit column sum: 28-bit limbs keep a column of up to 32 products plus the
+// incoming carry below 2^62, so no carry instruction is issued inside a
+// column. On gfx950 v_mad_u64_u32 issues at full rate, so carry fix-ups would
+// cost as much as the multiplies themselves.
+//
+// Lazy reduction: p < 2^446 so 4p < R; with inputs in [0, 2p) the Montgomery
+// output stays in [0, 2p) without a final subtraction. add/sub return [0, 2p).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gm {
+
+constexpr int NL = 16;
+constexpr uint32_t MASK28 = (1u << 28) - 1;
+
+// p in 28-bit limbs (little endian)
+constexpr uint32_t PL[NL] = {0xab0aaab, 0x1c0026a, 0x3b5cc31, 0x4528c57, 0xdee56ee, 0xc3e23ac,
+                             0x2824e6d, 0xbac7160, 0xf75a64b, 0x02fe320, 0x5a4b78a, 0xd640009,
+                             0x34965aa, 0x35200fc, 0xb28c5e5, 0x3cdee0f};
+// 2p in 28-bit limbs
+constexpr uint32_t P2L[NL] = {0x5615556, 0x38004d5, 0x76b9862, 0x8a518ae, 0xbdcaddc, 0x87c4759,
+                              0x5049cdb, 0x758e2c0, 0xeeb4c97, 0x05fc641, 0xb496f14, 0xac80012,
+                              0x692cb55, 0x6a401f8, 0x6518bca, 0x79bdc1f};
+// -p^{-1} mod 2^28
+constexpr uint32_t PINV28 = 0x035fffd;
+// R mod p (Montgomery one)
+constexpr uint32_t ONEL[NL] = {0x53d5554, 0x8fff655, 0x128cf3b, 0xeb5cea3, 0x846a446, 0xf07714c,
+                               0x5f6c648, 0x14e3a7f, 0x22966d1, 0xf40737c, 0x96d21d7, 0xa6fffda,
+                               0x2da6954, 0x2b7fc0f, 0x35ce86b, 0x0c847c1};
+// R^2 mod p
+constexpr uint32_t R2L[NL] = {0xc752b45, 0xff01ddd, 0xccf5b2a, 0x72a7421, 0xad692c7, 0x29c347d,
+                              0xe140eef, 0x22ea595, 0x5a1400c, 0x145218f, 0x91c9fec, 0x3d4b99d,
+                              0x37f9014, 0x4f3bb65, 0xc9be797, 0x3627854};
+
+struct Fp {
+    uint32_t v[NL];
+};
+
+struct Fp2 {
+    Fp c0, c1;
+};
+
+__host__ __device__ __forceinline__ void fp_set_const(Fp& r, const uint32_t (&c)[NL]) {
+#pragma unroll
+    for (int i = 0; i < NL; ++i) r.v[i] = c[i];
+}
+__host__ __device__ __forceinline__ void fp_set_zero(Fp& r) {
+#pragma unroll
+    for (int i = 0; i < NL; ++i) r.v[i] = 0;
+}
+__host__ __device__ __forceinline__ void fp_set_one(Fp& r) { fp_set_const(r, ONEL); }
+
+// r = a*b*R^{-1} mod p, inputs in [0, 2p), output in [0, 2p).
+__device__ __forceinline__ void fp_mul(Fp& r, const Fp& a, const Fp& b) {
+    uint32_t m[NL];
+    uint64_t acc = 0;
+#pragma unroll
+    for (int k = 0; k < NL; ++k) {
+#pragma unroll
+        for (int i = 0; i <= k; ++i) acc += (uint64_t)a.v[i] * b.v[k - i];
+#pragma unroll
+        for (int i = 0; i < k; ++i) acc += (uint64_t)m[i] * PL[k - i];
+        m[k] = ((uint32_t)acc * PINV28) & MASK28;
+        acc += (uint64_t)m[k] * PL[0];
+        acc >>= 28;
+    }
+#pragma unroll
+    for (int k = NL; k < 2 * NL - 1; ++k) {
+#pragma unroll
+        for (int i = k - NL + 1; i < NL; ++i) acc += (uint64_t)a.v[i] * b.v[k - i];
+#pragma unroll
+        for (int i = k - NL + 1; i < NL; ++i) acc += (uint64_t)m[i] * PL[k - i];
+        r.v[k - NL] = (uint32_t)acc & MASK28;
+        acc >>= 28;
+    }
+    r.v[NL - 1] = (uint32_t)acc;
+}
+
+// r = a^2 * R^{-1}: off-diagonal products once against doubled limbs.
+__device__ __forceinline__ void fp_sqr(Fp& r, const Fp& a) {
+    uint32_t m[NL], d[NL];
+#pragma unroll
+    for (int i = 0; i < NL; ++i) d[i] = a.v[i] << 1;
+    uint64_t acc = 0;
+#pragma unroll
+    for (int k = 0; k < NL; ++k) {
+#pragma unroll
+        for (int i = 0; i < k - i; ++i) acc += (uint64_t)d[i] * a.v[k - i];
+        if ((k & 1) == 0) acc += (uint64_t)a.v[k >> 1] * a.v[k >> 1];
+#pragma unroll
+        for (int i = 0; i < k; ++i) acc += (uint64_t)m[i] * PL[k - i];
+        m[k] = ((uint32_t)acc * PINV28) & MASK28;
+        acc += (uint64_t)m[k] * PL[0];
+        acc >>= 28;
+    }
+#pragma unroll
+    for (int k = NL; k < 2 * NL - 1; ++k) {
+#pragma unroll
+        for (int i = k - NL + 1; i < k - i; ++i) acc += (uint64_t)d[i] * a.v[k - i];
+        if ((k & 1) == 0) acc += (uint64_t)a.v[k >> 1] * a.v[k >> 1];
+#pragma unroll
+        for (int i = k - NL + 1; i < NL; ++i) acc += (uint64_t)m[i] * PL[k - i];
+        r.v[k - NL] = (uint32_t)acc & MASK28;
+        acc >>= 28;
+    }
+    r.v[NL - 1] = (uint32_t)acc;
+}
+
+// r = a + b mod p, [0, 2p) in and out.
+__device__ __forceinline__ void fp_add(Fp& r, const Fp& a, const Fp& b) {
+    uint32_t s[NL], d[NL];
+    uint32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+        uint32_t t = a.v[i] + b.v[i] + c;
+        s[i] = t & MASK28;
+        c = t >> 28;
+    }
+    int32_t br = 0;
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+        int32_t t = (int32_t)s[i] - (int32_t)P2L[i] + br;
+        d[i] = (uint32_t)t & MASK28;
+        br = t >> 28;
+    }
+    const bool keep = br < 0;
+#pragma unroll
+    for (int i = 0; i < NL; ++i) r.v[i] = keep ? s[i] : d[i];
+}
+
+// r = a - b mod p, [0, 2p) in and out.
+__device__ __forceinline__ void fp_sub(Fp& r, const Fp& a, const Fp& b) {
+    uint32_t d[NL];
+    int32_t br = 0;
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+        int32_t t = (int32_t)a.v[i] - (int32_t)b.v[i] + br;
+        d[i] = (uint32_t)t & MASK28;
+        br = t >> 28;
+    }
+    const uint32_t msk = (uint32_t)br;  // all ones when a < b
+    uint32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+        uint32_t t = d[i] + (P2L[i] & msk) + c;
+        r.v[i] = t & MASK28;
+        c = t >> 28;
+    }
+}
+
+__device__ __forceinline__ void fp_dbl(Fp& r, const Fp& a) { fp_add(r, a, a); }
+
+// r = -a mod p for a in [0, 2p): 2p - a in (0, 2p], folded to [0, 2p).
+__device__ __forceinline__ void fp_neg(Fp& r, const Fp& a) {
+    Fp z;
+    fp_set_zero(z);
+    fp_sub(r, z, a);
+}
+
+// Canonical representative in [0, p).
+__device__ __forceinline__ void fp_canon(Fp& r, const Fp& a) {
+    uint32_t d[NL];
+    int32_t br = 0;
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+        int32_t t = (int32_t)a.v[i] - (int32_t)PL[i] + br;
+        d[i] = (uint32_t)t & MASK28;
+        br = t >> 28;
+    }
+    const bool keep = br < 0;
+#pragma unroll
+    for (int i = 0; i < NL; ++i) r.v[i] = keep ? a.v[i] : d[i];
+}
+
+// a == 0 mod p for a in [0, 2p): a is 0 or p.
+__device__ __forceinline__ bool fp_is_zero(const Fp& a) {
+    uint32_t z = 0, q = 0;
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+        z |= a.v[i];
+        q |= a.v[i] ^ PL[i];
+    }
+    return z == 0 || q == 0;
+}
+
+// Exact zero of the representation (used for the infinity marker ZZ = 0).
+__device__ __forceinline__ bool fp_is_raw_zero(const Fp& a) {
+    uint32_t z = 0;
+#pragma unroll
+    for (int i = 0; i < NL; ++i) z |= a.v[i];
+    return z == 0;
+}
+
+__device__ __forceinline__ bool fp_eq(const Fp& a, const Fp& b) {
+    Fp d;
+    fp_sub(d, a, b);
+    return fp_is_zero(d);
+}
+
+// a^(p-2) (Fermat inverse); 0 maps to 0.
+__device__ inline void fp_inv(Fp& r, const Fp& a) {
+    // p - 2 in 32-bit words, most significant first
+    constexpr uint32_t E[14] = {0x3cdee0fb, 0x28c5e535, 0x200fc349, 0x65aad640, 0x0095a4b7, 0x8a02fe32, 0x0f75a64b,
+                                0xbac71602, 0x824e6dc3, 0xe23acdee, 0x56ee4528, 0xc573b5cc, 0x311c0026, 0xaab0aaa9};
+    Fp acc;
+    fp_set_one(acc);
+    for (int w = 0; w < 14; ++w) {
+        const uint32_t e = E[w];
+        for (int b = 31; b >= 0; --b) {
+            fp_sqr(acc, acc);
+            if ((e >> b) & 1) fp_mul(acc, acc, a);
+        }
+    }
+    r = acc;
+}
+
+// ------------------------------------------------------------------ Fp2
+// Fp2 = Fp[i]/(i^2 + 1) (non-residue -1 as in reference include/fp2.h:7).
+
+__device__ __forceinline__ void fp2_add(Fp2& r, const Fp2& a, const Fp2& b) {
+    fp_add(r.c0, a.c0, b.c0);
+    fp_add(r.c1, a.c1, b.c1);
+}
+__device__ __forceinline__ void fp2_sub(Fp2& r, const Fp2& a, const Fp2& b) {
+    fp_sub(r.c0, a.c0, b.c0);
+    fp_sub(r.c1, a.c1, b.c1);
+}
+__device__ __forceinline__ void fp2_dbl(Fp2& r, const Fp2& a) { fp2_add(r, a, a); }
+__device__ __forceinline__ void fp2_neg(Fp2& r, const Fp2& a) {
+    fp_neg(r.c0, a.c0);
+    fp_neg(r.c1, a.c1);
+}
+// Karatsuba: t0 = a0 b0, t1 = a1 b1, t2 = (a0+a1)(b0+b1); c0 = t0 - t1, c1 = t2 - t0 - t1.
+__device__ __forceinline__ void fp2_mul(Fp2& r, const Fp2& a, const Fp2& b) {
+    Fp t0, t1, t2, sa, sb;
+    fp_add(sa, a.c0, a.c1);
+    fp_add(sb, b.c0, b.c1);
+    fp_mul(t0, a.c0, b.c0);
+    fp_mul(t1, a.c1, b.c1);
+    fp_mul(t2, sa, sb);
+    fp_sub(r.c0, t0, t1);
+    fp_sub(t2, t2, t0);
+    fp_sub(r.c1, t2, t1);
+}
+// (a0 + a1 i)^2 = (a0 + a1)(a0 - a1) + 2 a0 a1 i
+__device__ __forceinline__ void fp2_sqr(Fp2& r, const Fp2& a) {
+    Fp s, d, t;
+    fp_add(s, a.c0, a.c1);
+    fp_sub(d, a.c0, a.c1);
+    fp_mul(t, a.c0, a.c1);
+    fp_mul(r.c0, s, d);
+    fp_dbl(r.c1, t);
+}
+__device__ __forceinline__ bool fp2_is_zero(const Fp2& a) { return fp_is_zero(a.c0) && fp_is_zero(a.c1); }
+__device__ __forceinline__ bool fp2_is_raw_zero(const Fp2& a) {
+    return fp_is_raw_zero(a.c0) && fp_is_raw_zero(a.c1);
+}
+__device__ __forceinline__ void fp2_canon(Fp2& r, const Fp2& a) {
+    fp_canon(r.c0, a.c0);
+    fp_canon(r.c1, a.c1);
+}
+__device__ __forceinline__ void fp2_inv(Fp2& r, const Fp2& a) {
+    Fp t0, t1, n;
+    fp_sqr(t0, a.c0);
+    fp_sqr(t1, a.c1);
+    fp_add(n, t0, t1);
+    fp_inv(n, n);
+    fp_mul(r.c0, a.c0, n);
+    fp_neg(t0, a.c1);
+    fp_mul(r.c1, t0, n);
+}
+
+// Field policies so the curve code is written once for G1 (Fp) and G2 (Fp2).
+struct FpOps {
+    using F = Fp;
+    static constexpr int WORDS = NL;
+    __device__ static __forceinline__ void add(F& r, const F& a, const F& b) { fp_add(r, a, b); }
+    __device__ static __forceinline__ void sub(F& r, const F& a, const F& b) { fp_sub(r, a, b); }
+    __device__ static __forceinline__ void dbl(F& r, const F& a) { fp_dbl(r, a); }
+    __device__ static __forceinline__ void neg(F& r, const F& a) { fp_neg(r, a); }
+    __device__ static __forceinline__ void mul(F& r, const F& a, const F& b) { fp_mul(r, a, b); }
+    __device__ static __forceinline__ void sqr(F& r, const F& a) { fp_sqr(r, a); }
+    __device__ static __forceinline__ bool is_zero(const F& a) { return fp_is_zero(a); }
+    __device__ static __forceinline__ bool is_raw_zero(const F& a) { return fp_is_raw_zero(a); }
+    __device__ static __forceinline__ void canon(F& r, const F& a) { fp_canon(r, a); }
+    __device__ static __forceinline__ void inv(F& r, const F& a) { fp_inv(r, a); }
+    __device__ static __forceinline__ void zero(F& r) { fp_set_zero(r); }
+    __device__ static __forceinline__ void one(F& r) { fp_set_one(r); }
+};
+
+struct Fp2Ops {
+    using F = Fp2;
+    static constexpr int WORDS = 2 * NL;
+    __device__ static __forceinline__ void add(F& r, const F& a, const F& b) { fp2_add(r, a, b); }
+    __device__ static __forceinline__ void sub(F& r, const F& a, const F& b) { fp2_sub(r, a, b); }
+    __device__ static __forceinline__ void dbl(F& r, const F& a) { fp2_dbl(r, a); }
+    __device__ static __forceinline__ void neg(F& r, const F& a) { fp2_neg(r, a); }
+    __device__ static __forceinline__ void mul(F& r, const F& a, const F& b) { fp2_mul(r, a, b); }
+    __device__ static __forceinline__ void sqr(F& r, const F& a) { fp2_sqr(r, a); }
+    __device__ static __forceinline__ bool is_zero(const F& a) { return fp2_is_zero(a); }
+    __device__ static __forceinline__ bool is_raw_zero(const F& a) { return fp2_is_raw_zero(a); }
+    __device__ static __forceinline__ void canon(F& r, const F& a) { fp2_canon(r, a); }
+    __device__ static __forceinline__ void inv(F& r, const F& a) { fp2_inv(r, a); }
+    __device__ static __forceinline__ void zero(F& r) {
+        fp_set_zero(r.c0);
+        fp_set_zero(r.c1);
+    }
+    __device__ static __forceinline__ void one(F& r) {
+        fp_set_one(r.c0);
+        fp_set_zero(r.c1);
+    }
+};
+
+}  // namespace gm

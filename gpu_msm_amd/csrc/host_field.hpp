@@ -1,0 +1,297 @@
+// Host-side BLS12-446 field and group arithmetic (x86-64, 7 x 64-bit limbs).
+//
+// Used by the library's host code: the final window combine of a single MSM
+// (a ~300-doubling serial chain that a CPU core runs far faster than one GPU
+// lane), normalisation to affine, and the ABI helpers of reference
+// rust-api/src/c_wrapper.cu:12-59,261-288 (affine<->projective, Montgomery
+// conversion). Values are canonical (< p), Montgomery form with R = 2^448, the
+// representation of reference src/fp.cu.
+#pragma once
+#include <stdint.h>
+#include <string.h>
+
+namespace gmh {
+
+typedef unsigned __int128 u128;
+
+struct Fp {
+    uint64_t l[7];
+};
+struct Fp2 {
+    Fp c0, c1;
+};
+
+static const Fp P = {{0x311c0026aab0aaabULL, 0x56ee4528c573b5ccULL, 0x824e6dc3e23acdeeULL, 0x0f75a64bbac71602ULL,
+                      0x0095a4b78a02fe32ULL, 0x200fc34965aad640ULL, 0x3cdee0fb28c5e535ULL}};
+static const Fp R2 = {{0x2AFF01DDDC752B45ULL, 0x92C772A7421CCF5BULL, 0x140EEF29C347DAD6ULL, 0xF5A1400C22EA595EULL,
+                       0x99D91C9FEC145218ULL, 0x3BB6537F90143D4BULL, 0x3627854C9BE7974FULL}};
+static const Fp ONE = {{0x3b8fff65553d5554ULL, 0xa446eb5cea3128cfULL, 0xf6c648f07714c846ULL, 0xc22966d114e3a7f5ULL,
+                        0xfda96d21d7f40737ULL, 0x7fc0f2da6954a6ffULL, 0x0c847c135ce86b2bULL}};
+static const uint64_t PINV = 0xcd63fd900035fffdULL;
+
+inline bool geq_p(const uint64_t* a) {
+    for (int i = 6; i >= 0; --i) {
+        if (a[i] != P.l[i]) return a[i] > P.l[i];
+    }
+    return true;
+}
+inline void sub_p(uint64_t* a) {
+    uint64_t br = 0;
+    for (int i = 0; i < 7; ++i) {
+        u128 d = (u128)a[i] - P.l[i] - br;
+        a[i] = (uint64_t)d;
+        br = (uint64_t)(d >> 64) & 1;
+    }
+}
+inline bool is_zero(const Fp& a) {
+    uint64_t o = 0;
+    for (int i = 0; i < 7; ++i) o |= a.l[i];
+    return o == 0;
+}
+inline bool eq(const Fp& a, const Fp& b) { return memcmp(a.l, b.l, sizeof a.l) == 0; }
+
+inline void add(Fp& r, const Fp& a, const Fp& b) {
+    uint64_t t[7];
+    u128 c = 0;
+    for (int i = 0; i < 7; ++i) {
+        c += (u128)a.l[i] + b.l[i];
+        t[i] = (uint64_t)c;
+        c >>= 64;
+    }
+    if (c || geq_p(t)) sub_p(t);
+    memcpy(r.l, t, sizeof t);
+}
+inline void sub(Fp& r, const Fp& a, const Fp& b) {
+    uint64_t t[7];
+    uint64_t br = 0;
+    for (int i = 0; i < 7; ++i) {
+        u128 d = (u128)a.l[i] - b.l[i] - br;
+        t[i] = (uint64_t)d;
+        br = (uint64_t)(d >> 64) & 1;
+    }
+    if (br) {
+        u128 c = 0;
+        for (int i = 0; i < 7; ++i) {
+            c += (u128)t[i] + P.l[i];
+            t[i] = (uint64_t)c;
+            c >>= 64;
+        }
+    }
+    memcpy(r.l, t, sizeof t);
+}
+inline void neg(Fp& r, const Fp& a) {
+    Fp z = {{0}};
+    sub(r, z, a);
+}
+// CIOS Montgomery product, canonical output.
+inline void mul(Fp& r, const Fp& a, const Fp& b) {
+    uint64_t t[9] = {0};
+    for (int i = 0; i < 7; ++i) {
+        u128 c = 0;
+        for (int j = 0; j < 7; ++j) {
+            c += (u128)a.l[j] * b.l[i] + t[j];
+            t[j] = (uint64_t)c;
+            c >>= 64;
+        }
+        c += t[7];
+        t[7] = (uint64_t)c;
+        t[8] = (uint64_t)(c >> 64);
+        uint64_t m = t[0] * PINV;
+        c = (u128)m * P.l[0] + t[0];
+        c >>= 64;
+        for (int j = 1; j < 7; ++j) {
+            c += (u128)m * P.l[j] + t[j];
+            t[j - 1] = (uint64_t)c;
+            c >>= 64;
+        }
+        c += t[7];
+        t[6] = (uint64_t)c;
+        t[7] = t[8] + (uint64_t)(c >> 64);
+    }
+    if (t[7] || geq_p(t)) sub_p(t);
+    memcpy(r.l, t, sizeof r.l);
+}
+inline void sqr(Fp& r, const Fp& a) { mul(r, a, a); }
+inline void dbl(Fp& r, const Fp& a) { add(r, a, a); }
+inline void to_mont(Fp& r, const Fp& a) { mul(r, a, R2); }
+inline void from_mont(Fp& r, const Fp& a) {
+    Fp one = {{1, 0, 0, 0, 0, 0, 0}};
+    mul(r, a, one);
+}
+// Fermat inverse a^(p-2); 0 -> 0.
+inline void inv(Fp& r, const Fp& a) {
+    if (is_zero(a)) {
+        r = a;
+        return;
+    }
+    Fp e = P;
+    e.l[0] -= 2;
+    Fp acc = ONE;
+    for (int i = 6; i >= 0; --i)
+        for (int b = 63; b >= 0; --b) {
+            sqr(acc, acc);
+            if ((e.l[i] >> b) & 1) mul(acc, acc, a);
+        }
+    r = acc;
+}
+// Reduce an arbitrary 448-bit value mod p (at most four subtractions since
+// 2^448 < 5p).
+inline void reduce(Fp& r, const Fp& a) {
+    r = a;
+    while (geq_p(r.l)) sub_p(r.l);
+}
+
+// ---------------------------------------------------------------- Fp2 (i^2 = -1)
+inline void add(Fp2& r, const Fp2& a, const Fp2& b) {
+    add(r.c0, a.c0, b.c0);
+    add(r.c1, a.c1, b.c1);
+}
+inline void sub(Fp2& r, const Fp2& a, const Fp2& b) {
+    sub(r.c0, a.c0, b.c0);
+    sub(r.c1, a.c1, b.c1);
+}
+inline void neg(Fp2& r, const Fp2& a) {
+    neg(r.c0, a.c0);
+    neg(r.c1, a.c1);
+}
+inline void dbl(Fp2& r, const Fp2& a) { add(r, a, a); }
+inline void mul(Fp2& r, const Fp2& a, const Fp2& b) {
+    Fp t0, t1, t2, t3;
+    mul(t0, a.c0, b.c0);
+    mul(t1, a.c1, b.c1);
+    add(t2, a.c0, a.c1);
+    add(t3, b.c0, b.c1);
+    mul(t2, t2, t3);
+    sub(r.c0, t0, t1);
+    sub(t2, t2, t0);
+    sub(r.c1, t2, t1);
+}
+inline void sqr(Fp2& r, const Fp2& a) { mul(r, a, a); }
+inline bool is_zero(const Fp2& a) { return is_zero(a.c0) && is_zero(a.c1); }
+inline void inv(Fp2& r, const Fp2& a) {
+    Fp t0, t1, n;
+    mul(t0, a.c0, a.c0);
+    mul(t1, a.c1, a.c1);
+    add(n, t0, t1);
+    inv(n, n);
+    mul(r.c0, a.c0, n);
+    neg(t0, a.c1);
+    mul(r.c1, t0, n);
+}
+
+// ------------------------------------------------------------- XYZZ group law
+template <class F>
+struct Xyzz {
+    F X, Y, ZZ, ZZZ;
+};
+
+template <class F>
+inline void set_inf(Xyzz<F>& p) {
+    memset(&p, 0, sizeof p);
+}
+template <class F>
+inline bool is_inf(const Xyzz<F>& p) {
+    return is_zero(p.ZZ);
+}
+
+template <class F>
+inline void xdbl(Xyzz<F>& r, const Xyzz<F>& p) {
+    if (is_inf(p)) {
+        r = p;
+        return;
+    }
+    F U, V, W, S, M, t;
+    dbl(U, p.Y);
+    sqr(V, U);
+    mul(W, U, V);
+    mul(S, p.X, V);
+    sqr(t, p.X);
+    dbl(M, t);
+    add(M, M, t);
+    mul(r.ZZ, V, p.ZZ);
+    mul(r.ZZZ, W, p.ZZZ);
+    mul(W, W, p.Y);
+    sqr(t, M);
+    sub(t, t, S);
+    sub(r.X, t, S);
+    sub(t, S, r.X);
+    mul(t, M, t);
+    sub(r.Y, t, W);
+}
+
+template <class F>
+inline void xadd(Xyzz<F>& r, const Xyzz<F>& p, const Xyzz<F>& q) {
+    if (is_inf(p)) {
+        r = q;
+        return;
+    }
+    if (is_inf(q)) {
+        r = p;
+        return;
+    }
+    F U1, S1, P_, R_, PP, PPP, Q, t;
+    mul(U1, p.X, q.ZZ);
+    mul(P_, q.X, p.ZZ);
+    sub(P_, P_, U1);
+    mul(S1, p.Y, q.ZZZ);
+    mul(R_, q.Y, p.ZZZ);
+    sub(R_, R_, S1);
+    if (is_zero(P_)) {
+        if (is_zero(R_))
+            xdbl(r, p);
+        else
+            set_inf(r);
+        return;
+    }
+    sqr(PP, P_);
+    mul(PPP, P_, PP);
+    mul(Q, U1, PP);
+    mul(t, p.ZZ, q.ZZ);
+    mul(r.ZZ, t, PP);
+    mul(t, p.ZZZ, q.ZZZ);
+    mul(r.ZZZ, t, PPP);
+    mul(S1, S1, PPP);
+    sqr(t, R_);
+    sub(t, t, PPP);
+    sub(t, t, Q);
+    sub(r.X, t, Q);
+    sub(t, Q, r.X);
+    mul(t, R_, t);
+    sub(r.Y, t, S1);
+}
+
+// (x, y) = (X/ZZ, Y/ZZZ); false for infinity.
+template <class F>
+inline bool to_affine(F& x, F& y, const Xyzz<F>& p) {
+    if (is_inf(p)) return false;
+    F t, iv;
+    mul(t, p.ZZ, p.ZZZ);
+    inv(iv, t);
+    mul(t, iv, p.ZZZ);
+    mul(x, p.X, t);
+    mul(t, iv, p.ZZ);
+    mul(y, p.Y, t);
+    return true;
+}
+
+// ------------------------------------------------- 28-bit device limb packing
+// Device elements are 16 x 28-bit limbs of the same 448-bit integer.
+inline void from_limbs28(Fp& r, const uint32_t* v) {
+    memset(r.l, 0, sizeof r.l);
+    for (int i = 0; i < 16; ++i) {
+        const int bit = 28 * i;
+        const uint64_t x = v[i] & 0xFFFFFFFu;
+        r.l[bit / 64] |= x << (bit % 64);
+        if (bit % 64 > 36) r.l[bit / 64 + 1] |= x >> (64 - bit % 64);
+    }
+}
+inline void to_limbs28(uint32_t* v, const Fp& a) {
+    for (int i = 0; i < 16; ++i) {
+        const int bit = 28 * i;
+        uint64_t x = a.l[bit / 64] >> (bit % 64);
+        if (bit % 64 > 36 && bit / 64 + 1 < 7) x |= a.l[bit / 64 + 1] << (64 - bit % 64);
+        v[i] = (uint32_t)x & 0xFFFFFFFu;
+    }
+}
+
+}  // namespace gmh

@@ -1,0 +1,292 @@
+// Pippenger MSM kernels for gfx950 (G1 and G2 share every kernel through the
+// field policy O).
+//
+// Pipeline for one MSM over N points with c-bit signed windows,
+// nw windows and B = 2^(c-1) buckets per window:
+//   k_digits_hist  signed-digit recoding of each scalar (reference
+//                  src/curve.cu:892-922 extracts unsigned windows; here digits
+//                  are in [-2^(c-1), 2^(c-1)] and the sign negates the base's
+//                  y), counting each (window, bucket) with an atomic whose
+//                  return value is the entry's rank inside its bucket
+//   scan           exclusive prefix sum of the counts (hipcub)
+//   k_scatter      writes point index | sign bit at offset[bucket] + rank,
+//                  so each bucket's points are contiguous
+//   k_accumulate   one lane per bucket: gathers its affine bases (128 B G1 /
+//                  256 B G2 records, one or two cache lines) and mixed-adds
+//                  them into an XYZZ accumulator
+//   k_bucket_reduce one lane per segment of buckets: running sums give
+//                  sum_b (b+1) B_b for the segment as T + lo * S
+//   k_window_sum   one workgroup per window: tree sum of the segment results
+//   combine        Horner over windows, most significant first (reference
+//                  src/curve.cu:2060-2179 combines on the CPU after every
+//                  window; here either on the host once, or on the device)
+#pragma once
+#include "curve.hpp"
+
+namespace gm {
+
+constexpr int MAX_SCALAR_LIMBS = 8;
+
+template <class T>
+__device__ __forceinline__ void ld16(T& dst, const T* src) {
+    static_assert(sizeof(T) % 16 == 0, "16-byte granularity");
+    const uint4* s = reinterpret_cast<const uint4*>(src);
+    uint4* d = reinterpret_cast<uint4*>(&dst);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 16); ++i) d[i] = s[i];
+}
+template <class T>
+__device__ __forceinline__ void st16(T* dst, const T& src) {
+    static_assert(sizeof(T) % 16 == 0, "16-byte granularity");
+    uint4* d = reinterpret_cast<uint4*>(dst);
+    const uint4* s = reinterpret_cast<const uint4*>(&src);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 16); ++i) d[i] = s[i];
+}
+
+// ------------------------------------------------------------ format import
+
+// 7 x u64 (any 448-bit value) -> 16 x 28-bit limbs, reduced mod p.
+__device__ __forceinline__ void fp_from_u64x7(Fp& r, const uint64_t* w) {
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+        const int bit = 28 * i;
+        uint64_t x = w[bit / 64] >> (bit % 64);
+        if (bit % 64 > 36) x |= w[bit / 64 + 1] << (64 - bit % 64);
+        r.v[i] = (uint32_t)x & MASK28;
+    }
+    // 2^448 < 5p: four conditional subtractions reach [0, p)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) fp_canon(r, r);
+}
+
+__device__ __forceinline__ void fp_to_u64x7(uint64_t* w, const Fp& a) {
+#pragma unroll
+    for (int j = 0; j < 7; ++j) w[j] = 0;
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+        const int bit = 28 * i;
+        const uint64_t x = a.v[i];
+        w[bit / 64] |= x << (bit % 64);
+        if (bit % 64 > 36) w[bit / 64 + 1] |= x >> (64 - bit % 64);
+    }
+}
+
+__device__ __forceinline__ void import_elem(Fp& r, const uint64_t* w, bool to_mont) {
+    fp_from_u64x7(r, w);
+    if (to_mont) {
+        Fp r2;
+        fp_set_const(r2, R2L);
+        fp_mul(r, r, r2);
+        fp_canon(r, r);
+    }
+}
+__device__ __forceinline__ void import_elem(Fp2& r, const uint64_t* w, bool to_mont) {
+    import_elem(r.c0, w, to_mont);
+    import_elem(r.c1, w + 7, to_mont);
+}
+__device__ __forceinline__ void export_elem(uint64_t* w, const Fp& a) {
+    Fp c;
+    fp_canon(c, a);
+    fp_to_u64x7(w, c);
+}
+__device__ __forceinline__ void export_elem(uint64_t* w, const Fp2& a) {
+    export_elem(w, a.c0);
+    export_elem(w + 7, a.c1);
+}
+
+template <class O>
+__device__ __forceinline__ void aff_set_inf(Aff<O>& p) {
+    O::zero(p.x);
+    O::zero(p.y);
+    reinterpret_cast<uint32_t*>(&p.x)[NL - 1] = INF_MARK;
+}
+
+// ABI affine records (reference include/curve.h:13-25) -> device affine.
+template <class O>
+__global__ void k_import_aff(Aff<O>* __restrict__ out, const uint8_t* __restrict__ in, int n, int to_mont) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    using L = AbiLayout<O>;
+    const uint8_t* rec = in + (size_t)i * L::AFF_BYTES;
+    const uint64_t* w = reinterpret_cast<const uint64_t*>(rec);
+    Aff<O> p;
+    if (rec[2 * L::ELEM_U64 * 8] != 0) {
+        aff_set_inf(p);
+    } else {
+        import_elem(p.x, w, to_mont != 0);
+        import_elem(p.y, w + L::ELEM_U64, to_mont != 0);
+    }
+    st16(&out[i], p);
+}
+
+// device affine -> ABI affine records, Montgomery form.
+template <class O>
+__global__ void k_export_aff(uint8_t* __restrict__ out, const Aff<O>* __restrict__ in, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    using L = AbiLayout<O>;
+    Aff<O> p;
+    ld16(p, &in[i]);
+    uint64_t w[2 * L::ELEM_U64 + 1];
+#pragma unroll
+    for (int j = 0; j < 2 * L::ELEM_U64 + 1; ++j) w[j] = 0;
+    if (aff_is_inf(p)) {
+        w[2 * L::ELEM_U64] = 1;  // bool infinity = true, padding zero
+    } else {
+        export_elem(w, p.x);
+        export_elem(w + L::ELEM_U64, p.y);
+    }
+    uint64_t* o = reinterpret_cast<uint64_t*>(out + (size_t)i * L::AFF_BYTES);
+#pragma unroll
+    for (int j = 0; j < 2 * L::ELEM_U64 + 1; ++j) o[j] = w[j];
+}
+
+// Bases P_i = k_i * G for 64-bit k_i (double-and-add, then affine).
+template <class O>
+__global__ void __launch_bounds__(64) k_gen_bases(Aff<O>* __restrict__ out, const uint64_t* __restrict__ k, int n,
+                                                  Aff<O> g) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t s = k[i];
+    Aff<O> r;
+    if (s == 0) {
+        aff_set_inf(r);
+    } else {
+        Xyzz<O> acc;
+        xyzz_set_inf(acc);
+        const int top = 63 - __builtin_clzll(s);
+        for (int b = top; b >= 0; --b) {
+            xyzz_dbl<O>(acc, acc);
+            if ((s >> b) & 1) xyzz_add_aff<O>(acc, g.x, g.y);
+        }
+        if (!xyzz_to_aff<O>(r.x, r.y, acc)) aff_set_inf(r);
+    }
+    st16(&out[i], r);
+}
+
+// ------------------------------------------------------ bucket accumulation
+
+template <class O>
+__global__ void __launch_bounds__(64) k_accumulate(const Aff<O>* __restrict__ bases, const uint32_t* __restrict__ sorted,
+                                                   const uint32_t* __restrict__ off, int nbuckets,
+                                                   Xyzz<O>* __restrict__ buckets) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nbuckets) return;
+    const uint32_t s = off[g], e = off[g + 1];
+    Xyzz<O> acc;
+    xyzz_set_inf(acc);
+    for (uint32_t j = s; j < e; ++j) {
+        const uint32_t v = sorted[j];
+        Aff<O> pt;
+        ld16(pt, &bases[v & 0x7FFFFFFFu]);
+        if (aff_is_inf(pt)) continue;
+        if (v >> 31) O::neg(pt.y, pt.y);
+        xyzz_add_aff<O>(acc, pt.x, pt.y);
+    }
+    st16(&buckets[g], acc);
+}
+
+// ----------------------------------------------------------- bucket reduce
+
+// One lane per (window, segment of seg buckets). Bucket b of a window holds
+// digit magnitude b+1. Output: sum_{b in seg} (b+1) B_b.
+template <class O>
+__global__ void __launch_bounds__(64) k_bucket_reduce(const Xyzz<O>* __restrict__ buckets, int nw, int B, int seg,
+                                                      Xyzz<O>* __restrict__ segs) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nseg = B / seg;
+    if (g >= nw * nseg) return;
+    const int w = g / nseg, t = g - w * nseg;
+    const Xyzz<O>* bw = buckets + (size_t)w * B + (size_t)t * seg;
+    Xyzz<O> S, T;
+    xyzz_set_inf(S);
+    xyzz_set_inf(T);
+    for (int b = seg - 1; b >= 0; --b) {
+        Xyzz<O> bk;
+        ld16(bk, &bw[b]);
+        xyzz_add<O>(S, S, bk);
+        xyzz_add<O>(T, T, S);
+    }
+    const uint32_t lo = (uint32_t)t * seg;
+    if (lo != 0 && !xyzz_is_inf(S)) {
+        Xyzz<O> M;
+        xyzz_set_inf(M);
+        const int top = 31 - __builtin_clz(lo);
+        for (int b = top; b >= 0; --b) {
+            xyzz_dbl<O>(M, M);
+            if ((lo >> b) & 1) xyzz_add<O>(M, M, S);
+        }
+        xyzz_add<O>(T, T, M);
+    }
+    st16(&segs[g], T);
+}
+
+// One workgroup per window: tree sum of nseg segment results.
+template <class O, int THREADS>
+__global__ void __launch_bounds__(THREADS) k_window_sum(const Xyzz<O>* __restrict__ segs, int nseg,
+                                                        Xyzz<O>* __restrict__ wsum) {
+    __shared__ Xyzz<O> sh[THREADS];
+    const int w = blockIdx.x, tid = threadIdx.x;
+    Xyzz<O> acc;
+    xyzz_set_inf(acc);
+    for (int t = tid; t < nseg; t += THREADS) {
+        Xyzz<O> v;
+        ld16(v, &segs[(size_t)w * nseg + t]);
+        xyzz_add<O>(acc, acc, v);
+    }
+    sh[tid] = acc;
+    __syncthreads();
+    for (int s = THREADS / 2; s > 0; s >>= 1) {
+        if (tid < s) {
+            Xyzz<O> a = sh[tid], b = sh[tid + s];
+            xyzz_add<O>(a, a, b);
+            sh[tid] = a;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) st16(&wsum[w], sh[0]);
+}
+
+// Horner over window sums on the device, one lane per MSM (window sums of MSM
+// m at wsum[m * nw ...]); writes the canonical affine result of each MSM.
+template <class O>
+__global__ void __launch_bounds__(64) k_horner(const Xyzz<O>* __restrict__ wsum, int count, int nw, int c,
+                                               Aff<O>* __restrict__ out) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= count) return;
+    const Xyzz<O>* ws = wsum + (size_t)m * nw;
+    Xyzz<O> r;
+    ld16(r, &ws[nw - 1]);
+    for (int w = nw - 2; w >= 0; --w) {
+        for (int k = 0; k < c; ++k) xyzz_dbl<O>(r, r);
+        Xyzz<O> v;
+        ld16(v, &ws[w]);
+        xyzz_add<O>(r, r, v);
+    }
+    Aff<O> a;
+    if (!xyzz_to_aff<O>(a.x, a.y, r)) aff_set_inf(a);
+    st16(&out[m], a);
+}
+
+// device affine -> ABI projective (X, Y, Z) = (x, y, R mod p) or (0, 0, 0),
+// the representation reference src/curve.cu:396-432 gives an affine point.
+template <class O>
+__global__ void __launch_bounds__(64) k_aff_to_abi_proj(uint8_t* __restrict__ out, const Aff<O>* __restrict__ in,
+                                                        int count) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= count) return;
+    using L = AbiLayout<O>;
+    Aff<O> p;
+    ld16(p, &in[m]);
+    uint64_t* o = reinterpret_cast<uint64_t*>(out + (size_t)m * L::PROJ_BYTES);
+    for (int j = 0; j < 3 * L::ELEM_U64; ++j) o[j] = 0;
+    if (aff_is_inf(p)) return;
+    export_elem(o, p.x);
+    export_elem(o + L::ELEM_U64, p.y);
+    typename O::F one;
+    O::one(one);
+    export_elem(o + 2 * L::ELEM_U64, one);
+}
+
+}  // namespace gm

@@ -1,0 +1,81 @@
+/*
+ * Extensions of the BLS12-446 MSM C ABI beyond the reference's
+ * rust_wrapper.h. Each has no direct counterpart in the reference unless its
+ * comment cites one. Return codes follow the reference wrappers
+ * (rust-api/src/c_wrapper.cu:71-153): 0 ok, -1 bad size/argument, -2 device
+ * setup failed, -3 allocation failed, -4 HIP error; plus -5 scalar does not
+ * fit the declared bit bound.
+ */
+#ifndef GPU_MSM_EXT_H
+#define GPU_MSM_EXT_H
+
+#include <stddef.h>
+
+#include "gpu_msm_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Input-form and scalar flags for the *_msm_multi_wrapper entry points. */
+#define GM_POINTS_MONTGOMERY 0x1u  /* points already in Montgomery form        */
+#define GM_SCALARS_LT_R      0x2u  /* every scalar < r (299 bits): fewer windows */
+
+#define GM_CURVE_G1 1
+#define GM_CURVE_G2 2
+
+/* Multi-limb MSM with host pointers: the semantics of the reference's
+ * point_msm<T> (src/curve.cu:2763-2826, scalars at stride scalar_limbs
+ * little-endian u64 limbs), exported with an explicit input form so G1 and G2
+ * callers are symmetric. scalar_limbs in [1, 8]. Output projective,
+ * Montgomery form, Z = R mod p or the all-zero point. */
+int g1_msm_multi_wrapper(G1ProjectivePoint* result, const G1Point* points, const uint64_t* scalars,
+                         int scalar_limbs, int n, uint32_t gpu_index, uint32_t flags);
+int g2_msm_multi_wrapper(G2ProjectivePoint* result, const G2Point* points, const uint64_t* scalars,
+                         int scalar_limbs, int n, uint32_t gpu_index, uint32_t flags);
+
+/* ---------------------------------------------------------------- plans */
+
+/* Window width and workspace the engine would use (window <= 0: heuristic). */
+int gm_plan_query(int curve, int n, int scalar_limbs, int scalar_bits, int window, int* c_out, int* windows_out,
+                  size_t* workspace_bytes_out);
+
+/* ------------------------------------------------- resident device data */
+
+typedef struct gm_bases gm_bases;     /* affine bases resident in HBM      */
+typedef struct gm_scalars gm_scalars; /* scalars resident in HBM           */
+
+/* Upload ABI points (G1Point / G2Point records), converting to the engine's
+ * resident format. montgomery != 0: points already in Montgomery form. */
+int gm_bases_upload(gm_bases** out, int curve, const void* host_points, int n, int montgomery, uint32_t gpu_index);
+/* Bases P_i = k_i * G computed on the device from 64-bit k_i (host array). */
+int gm_bases_from_k(gm_bases** out, int curve, const uint64_t* host_k, int n, uint32_t gpu_index);
+/* Copy bases back as ABI records (Montgomery form). */
+int gm_bases_download(const gm_bases* b, void* host_points, int offset, int n);
+int gm_bases_count(const gm_bases* b);
+void gm_bases_free(gm_bases* b);
+
+int gm_scalars_upload(gm_scalars** out, const uint64_t* host, int n, int limbs, uint32_t gpu_index);
+void gm_scalars_free(gm_scalars* s);
+
+/* MSM over resident data: bases[base_offset .. +n) with scalars
+ * [scalar_offset .. +n). Output: affine ABI record (G1Point / G2Point),
+ * Montgomery form. scalar_bits: bound on scalar values (0 = 64 * limbs).
+ * window <= 0 picks the heuristic. device_combine != 0 runs the final
+ * window combine on the GPU instead of the host. */
+int gm_msm_resident(void* result_affine, const gm_bases* b, int base_offset, const gm_scalars* s, int scalar_offset,
+                    int n, int scalar_bits, int window, int device_combine);
+
+/* -------------------------------------------------------------- profiling */
+
+/* Record HIP events between pipeline stages of subsequent MSMs on this GPU. */
+int gm_set_profiling(uint32_t gpu_index, int on);
+/* Stage times (ms) of the last profiled MSM: [hist, scan, scatter,
+ * accumulate, bucket_reduce, window_sum]. */
+int gm_profile_last(uint32_t gpu_index, float* stage_ms, int count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPU_MSM_EXT_H */

@@ -1,0 +1,220 @@
+"""GPU parity tests: the HIP MSM engine (through the C ABI) against the CPU
+oracle, the reference tests' closed forms and the committed golden vectors.
+Comparison is bit-exact on the affine result in Montgomery form."""
+import numpy as np
+import pytest
+
+import gpu_msm_amd as gm
+import pymodel as m
+from oracle import oracle as o
+
+pytestmark = pytest.mark.gpu
+
+
+def aff_g1(rec):
+    return m.g1_from_abi(rec)
+
+
+def proj_to_aff(curve, proj):
+    out = np.zeros(1, dtype=gm.G1_DTYPE if curve == "g1" else gm.G2_DTYPE)
+    fn = "projective_to_affine_g1_wrapper" if curve == "g1" else "projective_to_affine_g2_wrapper"
+    getattr(gm.lib(), fn)(out.ctypes.data, proj.ctypes.data)
+    return out
+
+
+def golden_point(curve, rec):
+    if rec is None:
+        return None
+    if curve == "g1":
+        return (int(rec[0], 16), int(rec[1], 16))
+    return ((int(rec[0][0], 16), int(rec[0][1], 16)), (int(rec[1][0], 16), int(rec[1][1], 16)))
+
+
+def as_point(curve, rec):
+    return m.g1_from_abi(rec) if curve == "g1" else m.g2_from_abi(rec)
+
+
+def same(a, b):
+    return a.tobytes() == b.tobytes()
+
+
+# ----------------------------------------------------------------- ABI wrappers
+
+def test_g1_msm_wrapper_u64_vs_oracle():
+    n = 300
+    ks = o.gen_u64(77, n)
+    bases = o.bases_from_k("g1", ks)
+    bases[17]["inf"] = 1
+    bases[17]["x"] = 0
+    bases[17]["y"] = 0
+    sc = o.gen_u64(78, n)
+    sc[3] = 0
+    sc[4] = 1
+    std = np.zeros_like(bases)
+    for i in range(n):
+        std[i] = o.from_mont("g1", bases[i:i + 1])[0]
+    res = gm.G1Projective.msm(std, sc)
+    got = res.to_affine().rec
+    want = o.msm("g1", bases, sc.reshape(-1, 1))
+    assert same(got, want)
+
+
+def test_g2_msm_wrapper_u64_vs_oracle():
+    n = 100
+    ks = o.gen_u64(79, n)
+    bases = o.bases_from_k("g2", ks)
+    sc = o.gen_u64(80, n)
+    res = gm.G2Projective.msm(bases, sc)  # G2 points taken as Montgomery form
+    got = res.to_affine().rec
+    want = o.msm("g2", bases, sc.reshape(-1, 1))
+    assert same(got, want)
+
+
+@pytest.mark.parametrize("curve,n", [("g1", 10), ("g1", 100), ("g2", 10)])
+def test_reference_triangular_closed_form(golden, curve, n):
+    """reference tests/test_msm.cu:132-592: bases G, scalars 1..N."""
+    g = o.g1_generator(mont=(curve == "g2")) if curve == "g1" else o.g2_generator(mont=True)
+    pts = np.repeat(g, n)
+    sc = np.arange(1, n + 1, dtype=np.uint64)
+    cls = gm.G1Projective if curve == "g1" else gm.G2Projective
+    res = cls.msm(pts, sc).to_affine().rec
+    assert as_point(curve, res[0]) == golden_point(curve, golden["triangular"][f"{curve}_{n}"])
+
+
+def test_c1_host_config_g1_1024(golden):
+    """C1: G1, N=1024, bases k_i*G, scalars in [0, r) with 8 zeros and 8 ones."""
+    c = golden["c1_g1_1024"]
+    n = c["n"]
+    ks = o.gen_u64(c["seed_k"], n)
+    ss = o.gen_scalars_modr(c["seed_s"], n)
+    ss[c["zero_idx"]] = 0
+    for i in c["one_idx"]:
+        ss[i] = [1, 0, 0, 0, 0]
+    bases = o.bases_from_k("g1", ks)
+    res = gm.G1Projective.msm_multi(bases, ss, montgomery=True, scalars_lt_r=True).to_affine().rec
+    assert as_point("g1", res[0]) == golden_point("g1", c["result"])
+    assert same(res, o.msm("g1", bases, ss))
+
+
+def test_edge_cases_g1():
+    g = o.g1_generator()
+    ng = g.copy()
+    ng[0]["y"] = o.fp_unop("or_fp_neg", g[0]["y"].copy())
+    two = o.scalar_mul("g1", g, 2)
+    # P + (-P) = O; all-zero scalars = O; duplicate points take the doubling path
+    pts = np.concatenate([g, ng])
+    r = gm.G1Projective.msm_multi(pts, np.array([[5], [5]], dtype=np.uint64), montgomery=True).rec
+    assert not r.view(np.uint64).any()
+    r = gm.G1Projective.msm_multi(pts, np.zeros((2, 1), dtype=np.uint64), montgomery=True).rec
+    assert not r.view(np.uint64).any()
+    pts = np.concatenate([g, g, g, two])
+    r = gm.G1Projective.msm_multi(pts, np.array([[1], [1], [1], [3]], dtype=np.uint64), montgomery=True)
+    assert as_point("g1", r.to_affine().rec[0]) == m.ec_mul(m.FP_OPS, m.G1_STD, 9)
+    # n = 1, scalar r - 1 (-G)
+    sc = np.array([o.int_to_limbs(m.R_ORDER - 1, 5)], dtype=np.uint64)
+    r = gm.G1Projective.msm_multi(g, sc, montgomery=True, scalars_lt_r=True)
+    assert as_point("g1", r.to_affine().rec[0]) == m.ec_neg(m.FP_OPS, m.G1_STD)
+    # scalar bigger than the declared bound is rejected, not silently wrapped
+    big = np.array([o.int_to_limbs(1 << 310, 5)], dtype=np.uint64)
+    with pytest.raises(gm.MsmError):
+        gm.G1Projective.msm_multi(g, big, montgomery=True, scalars_lt_r=True)
+    # full 320-bit scalars are accepted without the flag
+    r = gm.G1Projective.msm_multi(g, big, montgomery=True)
+    assert as_point("g1", r.to_affine().rec[0]) == m.ec_mul(m.FP_OPS, m.G1_STD, 1 << 310)
+
+
+def test_edge_cases_g2_infinity_and_zero():
+    ks = o.gen_u64(81, 40)
+    bases = o.bases_from_k("g2", ks)
+    bases[0]["inf"] = 1
+    bases[0]["x"] = 0
+    bases[0]["y"] = 0
+    ss = o.gen_scalars_modr(82, 40)
+    ss[5] = 0
+    r = gm.G2Projective.msm_multi(bases, ss, montgomery=True, scalars_lt_r=True).to_affine().rec
+    assert same(r, o.msm("g2", bases, ss))
+
+
+# -------------------------------------------------------------- resident engine
+
+@pytest.fixture(scope="module")
+def g1_pool_16(golden):
+    w = golden["g1_2_16"]
+    ks = o.gen_u64(w["seed_k"], w["n"])
+    ss = o.gen_scalars_modr(w["seed_s"], w["n"])
+    b = gm.Bases.from_k(gm.CURVE_G1, ks)
+    s = gm.Scalars.upload(ss)
+    return ks, ss, b, s
+
+
+def test_generated_bases_match_oracle(g1_pool_16):
+    ks, _, b, _ = g1_pool_16
+    got = b.download(0, 512)
+    want = o.bases_from_k("g1", ks[:512])
+    assert same(got, want)
+    got = b.download(60000, 64)
+    assert same(got, o.bases_from_k("g1", ks[60000:60064]))
+
+
+def test_g1_2_16_full_and_windows(golden, g1_pool_16):
+    ks, ss, b, s = g1_pool_16
+    w = golden["g1_2_16"]
+    ref = gm.msm_resident(b, s, scalar_bits=299)
+    assert as_point("g1", ref[0]) == golden_point("g1", w["result"])
+    for c in (4, 9, 13, 16):
+        assert same(gm.msm_resident(b, s, scalar_bits=299, window=c), ref), c
+    assert same(gm.msm_resident(b, s, scalar_bits=299, device_combine=True), ref)
+    # subset vs the oracle's own Pippenger
+    sub = gm.msm_resident(b, s, n=1024, base_offset=4096, scalar_offset=4096, scalar_bits=299)
+    assert same(sub, o.msm("g1", o.bases_from_k("g1", ks[4096:5120]), ss[4096:5120]))
+
+
+def test_g2_2_14_closed_form_with_zeros(golden):
+    w = golden["g2_2_14"]
+    n = w["n"]
+    ks = o.gen_u64(w["seed_k"], n)
+    ss = o.gen_scalars_modr(w["seed_s"], n)
+    zs = []
+    gz = m.MT64(w["zero_seed"])
+    seen = set()
+    while len(zs) < n // 100:
+        v = gz.next() % n
+        if v not in seen:
+            seen.add(v)
+            zs.append(v)
+    ss[zs] = 0
+    b = gm.Bases.from_k(gm.CURVE_G2, ks)
+    s = gm.Scalars.upload(ss)
+    assert same(b.download(0, 64), o.bases_from_k("g2", ks[:64]))
+    r = gm.msm_resident(b, s, scalar_bits=299)
+    assert as_point("g2", r[0]) == golden_point("g2", w["result"])
+    sub = gm.msm_resident(b, s, n=256, base_offset=100, scalar_offset=100, scalar_bits=299)
+    assert same(sub, o.msm("g2", o.bases_from_k("g2", ks[100:356]), ss[100:356]))
+
+
+@pytest.mark.parametrize("curve,lg", [("g1", 18), ("g1", 20), ("g2", 16), ("g2", 20)])
+def test_full_size_closed_form(golden, curve, lg):
+    """Full-size configs: the closed form (sum s_i k_i mod r) * G pins the
+    whole affine result; a 1024-point subset is recomputed by the oracle."""
+    w = golden[f"{curve}_2_{lg}"]
+    n = w["n"]
+    ks = o.gen_u64(w["seed_k"], n)
+    ss = o.gen_scalars_modr(w["seed_s"], n)
+    if curve == "g2":
+        zs, seen = [], set()
+        gz = m.MT64(w["zero_seed"])
+        while len(zs) < n // 100:
+            v = gz.next() % n
+            if v not in seen:
+                seen.add(v)
+                zs.append(v)
+        ss[zs] = 0
+    assert hex(o.limbs_to_int(ss[-1])) == w["s_last"]
+    cv = gm.CURVE_G1 if curve == "g1" else gm.CURVE_G2
+    b = gm.Bases.from_k(cv, ks)
+    s = gm.Scalars.upload(ss)
+    r = gm.msm_resident(b, s, scalar_bits=299)
+    assert as_point(curve, r[0]) == golden_point(curve, w["result"])
+    off = n // 2 + 3
+    sub = gm.msm_resident(b, s, n=1024, base_offset=off, scalar_offset=off, scalar_bits=299)
+    assert same(sub, o.msm(curve, o.bases_from_k(curve, ks[off:off + 1024]), ss[off:off + 1024]))
