@@ -53,7 +53,7 @@ EXPORTS = [
     "g1_msm_multi_wrapper", "g2_msm_multi_wrapper", "gm_plan_query",
     "gm_bases_upload", "gm_bases_from_k", "gm_bases_download", "gm_bases_count", "gm_bases_free",
     "gm_scalars_upload", "gm_scalars_free", "gm_msm_resident",
-    "gm_set_profiling", "gm_profile_last",
+    "gm_set_profiling", "gm_profile_last", "gm_last_msm_info",
 ]
 
 
@@ -102,6 +102,7 @@ def lib() -> ctypes.CDLL:
         "gm_msm_resident": ([vp, vp, i, vp, i, i, i, i, i], i),
         "gm_set_profiling": ([u32, i], i),
         "gm_profile_last": ([u32, ctypes.POINTER(ctypes.c_float), i], i),
+        "gm_last_msm_info": ([u32, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(ctypes.c_longlong)], i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -357,3 +358,9 @@ def profile_last(gpu_index: int = 0) -> dict:
     buf = (ctypes.c_float * len(STAGES))()
     _check(lib().gm_profile_last(gpu_index, buf, len(STAGES)), "profile read")
     return dict(zip(STAGES, list(buf)))
+
+
+def last_msm_info(gpu_index: int = 0) -> dict:
+    c, nw, e = ctypes.c_int(), ctypes.c_int(), ctypes.c_longlong()
+    _check(lib().gm_last_msm_info(gpu_index, ctypes.byref(c), ctypes.byref(nw), ctypes.byref(e)), "msm info")
+    return {"window": c.value, "windows": nw.value, "entries": e.value}

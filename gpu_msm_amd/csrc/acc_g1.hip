@@ -6,10 +6,17 @@ namespace gm {
 
 template <class O>
 hipError_t launch_accumulate(hipStream_t s, const Plan& p, const Aff<O>* d_bases, uint8_t* ws) {
-    const size_t nbk = (size_t)p.nw * p.B;
-    k_accumulate<O><<<blocks_for((long)nbk, 64), 64, 0, s>>>(
-        d_bases, reinterpret_cast<const uint32_t*>(ws + p.o_sorted), reinterpret_cast<const uint32_t*>(ws + p.o_off),
-        (int)nbk, reinterpret_cast<Xyzz<O>*>(ws + p.o_buckets));
+    const int nbk = p.nw * p.B;
+    const uint32_t* off = reinterpret_cast<const uint32_t*>(ws + p.o_off);
+    Xyzz<O>* buckets = reinterpret_cast<Xyzz<O>*>(ws + p.o_buckets);
+    Xyzz<O>* head = reinterpret_cast<Xyzz<O>*>(ws + p.o_head);
+    Xyzz<O>* tail = reinterpret_cast<Xyzz<O>*>(ws + p.o_tail);
+    if (p.lanes > 0) {
+        k_accumulate<O><<<blocks_for(p.lanes, 64), 64, 0, s>>>(
+            d_bases, reinterpret_cast<const uint64_t*>(ws + p.o_sorted), off, nbk, p.chunk, buckets, head, tail);
+        GM_TRY(hipGetLastError());
+    }
+    k_bucket_fixup<O><<<blocks_for(nbk, 64), 64, 0, s>>>(off, nbk, p.chunk, head, tail, buckets);
     return hipGetLastError();
 }
 

@@ -75,6 +75,8 @@ bool make_plan(Plan& p, int curve, int n, int L, int bits, int c) {
     p.B = 1 << (c - 1);
     p.seg = seg_for(p.B);
     p.nseg = p.B / p.seg;
+    p.chunk = 32;
+    p.lanes = (int)(((size_t)p.nw * n + p.chunk - 1) / p.chunk);
     const size_t nbk = (size_t)p.nw * p.B;
     size_t cub = 0;
     if (hipcub::DeviceScan::ExclusiveSum(nullptr, cub, (uint32_t*)nullptr, (uint32_t*)nullptr, (int)(nbk + 1)) !=
@@ -93,9 +95,13 @@ bool make_plan(Plan& p, int curve, int n, int L, int bits, int c) {
     p.o_rank = o;
     o += al((size_t)p.nw * n * 4 + 4);
     p.o_sorted = o;
-    o += al((size_t)p.nw * n * 4 + 4);
+    o += al((size_t)p.nw * n * 8 + 8);
     p.o_buckets = o;
     o += al(nbk * xb);
+    p.o_head = o;
+    o += al((size_t)p.lanes * xb);
+    p.o_tail = o;
+    o += al((size_t)p.lanes * xb);
     p.o_segs = o;
     o += al((size_t)p.nw * p.nseg * xb);
     p.o_wsum = o;
@@ -126,6 +132,9 @@ struct DevCtx {
     std::mutex mu;
     gm::Prof prof{};
     bool prof_valid = false;
+    // last MSM: window, windows, (window, bucket) entries accumulated
+    int last_c = 0, last_nw = 0;
+    long long last_entries = 0;
 };
 
 std::mutex g_ctx_mu;
@@ -300,6 +309,9 @@ int run_msm_locked(DevCtx* c, const Aff<O>* d_bases, const uint64_t* d_scalars, 
     uint8_t* h = c->pinned;
     if (hipMemcpyAsync(h + wsb + sizeof(Aff<O>), ws + p.o_err, 4, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
         return -4;
+    if (hipMemcpyAsync(h + wsb + sizeof(Aff<O>) + 4, ws + p.o_off + (size_t)p.nw * p.B * 4, 4, hipMemcpyDeviceToHost,
+                       c->stream) != hipSuccess)
+        return -4;
     if (device_combine) {
         if (gm::Engine<O>::device_combine(c->stream, p, ws) != hipSuccess) return -4;
         if (hipMemcpyAsync(h + wsb, ws + p.o_out, sizeof(Aff<O>), hipMemcpyDeviceToHost, c->stream) != hipSuccess)
@@ -308,9 +320,13 @@ int run_msm_locked(DevCtx* c, const Aff<O>* d_bases, const uint64_t* d_scalars, 
         if (hipMemcpyAsync(h, ws + p.o_wsum, wsb, hipMemcpyDeviceToHost, c->stream) != hipSuccess) return -4;
     }
     if (hipStreamSynchronize(c->stream) != hipSuccess) return -4;
-    uint32_t err;
+    uint32_t err, entries;
     memcpy(&err, h + wsb + sizeof(Aff<O>), 4);
+    memcpy(&entries, h + wsb + sizeof(Aff<O>) + 4, 4);
     if (err) return -5;
+    c->last_c = p.c;
+    c->last_nw = p.nw;
+    c->last_entries = entries;
     if (device_combine) {
         const uint32_t* a = reinterpret_cast<const uint32_t*>(h + wsb);
         constexpr int W = words_of<HF>();
@@ -718,6 +734,16 @@ int gm_set_profiling(uint32_t gpu_index, int on) {
     std::lock_guard<std::mutex> lk(c->mu);
     c->prof.on = on != 0;
     c->prof_valid = false;
+    return 0;
+}
+
+int gm_last_msm_info(uint32_t gpu_index, int* window, int* windows, long long* entries) {
+    DevCtx* c = get_ctx(gpu_index);
+    if (!c) return -2;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (window) *window = c->last_c;
+    if (windows) *windows = c->last_nw;
+    if (entries) *entries = c->last_entries;
     return 0;
 }
 

@@ -47,9 +47,11 @@ struct Plan {
     int B;        // buckets per window = 2^(c-1)
     int seg;      // buckets per bucket-reduce lane
     int nseg;     // segments per window
+    int chunk;    // entries per accumulation lane
+    int lanes;    // accumulation lanes (upper bound: all digits non-zero)
     size_t cub_bytes;
     // workspace offsets (bytes)
-    size_t o_cnt, o_off, o_rank, o_sorted, o_buckets, o_segs, o_wsum, o_out, o_err, o_cub, total;
+    size_t o_cnt, o_off, o_rank, o_sorted, o_buckets, o_head, o_tail, o_segs, o_wsum, o_out, o_err, o_cub, total;
 };
 
 size_t xyzz_bytes(int curve);

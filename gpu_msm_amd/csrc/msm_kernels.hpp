@@ -167,22 +167,76 @@ __global__ void __launch_bounds__(64) k_gen_bases(Aff<O>* __restrict__ out, cons
 
 // ------------------------------------------------------ bucket accumulation
 
+// Entries are 64-bit (global bucket << 32) | point index | sign << 31, sorted
+// so each bucket's entries are contiguous (offsets in off[]). Every lane takes
+// a fixed chunk of K consecutive entries, whatever the bucket sizes, so lanes
+// stay balanced even when a window has few, very large buckets (the top window
+// of a 299-bit scalar holds only ~11 bits). A run of one bucket that lies
+// wholly inside a chunk is written to its bucket; a run cut by the chunk's
+// start goes to head[lane], one cut only by its end to tail[lane], and
+// k_bucket_fixup sums those partials.
 template <class O>
-__global__ void __launch_bounds__(64) k_accumulate(const Aff<O>* __restrict__ bases, const uint32_t* __restrict__ sorted,
-                                                   const uint32_t* __restrict__ off, int nbuckets,
-                                                   Xyzz<O>* __restrict__ buckets) {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= nbuckets) return;
-    const uint32_t s = off[g], e = off[g + 1];
+__global__ void __launch_bounds__(64) k_accumulate(const Aff<O>* __restrict__ bases, const uint64_t* __restrict__ entries,
+                                                   const uint32_t* __restrict__ off, int nbk, int K,
+                                                   Xyzz<O>* __restrict__ buckets, Xyzz<O>* __restrict__ head,
+                                                   Xyzz<O>* __restrict__ tail) {
+    const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t T = off[nbk];
+    const uint32_t j0 = l * (uint32_t)K;
+    if (j0 >= T) return;
+    const uint32_t j1 = min(j0 + (uint32_t)K, T);
     Xyzz<O> acc;
     xyzz_set_inf(acc);
-    for (uint32_t j = s; j < e; ++j) {
-        const uint32_t v = sorted[j];
+    uint32_t cur = (uint32_t)(entries[j0] >> 32);
+    bool first = true;
+    for (uint32_t j = j0; j < j1; ++j) {
+        const uint64_t e = entries[j];
+        const uint32_t g = (uint32_t)(e >> 32);
+        if (g != cur) {
+            if (first && off[cur] < j0)
+                st16(&head[l], acc);
+            else
+                st16(&buckets[cur], acc);
+            xyzz_set_inf(acc);
+            cur = g;
+            first = false;
+        }
         Aff<O> pt;
-        ld16(pt, &bases[v & 0x7FFFFFFFu]);
+        ld16(pt, &bases[(uint32_t)e & 0x7FFFFFFFu]);
         if (aff_is_inf(pt)) continue;
-        if (v >> 31) O::neg(pt.y, pt.y);
+        if ((uint32_t)e >> 31) O::neg(pt.y, pt.y);
         xyzz_add_aff<O>(acc, pt.x, pt.y);
+    }
+    if (first && off[cur] < j0)
+        st16(&head[l], acc);
+    else if (off[cur + 1] > j1)
+        st16(&tail[l], acc);
+    else
+        st16(&buckets[cur], acc);
+}
+
+// One lane per bucket: empty buckets become O; a bucket cut across chunks is
+// tail[first lane] + head[every later lane it reaches].
+template <class O>
+__global__ void __launch_bounds__(64) k_bucket_fixup(const uint32_t* __restrict__ off, int nbk, int K,
+                                                     const Xyzz<O>* __restrict__ head,
+                                                     const Xyzz<O>* __restrict__ tail, Xyzz<O>* __restrict__ buckets) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nbk) return;
+    const uint32_t s = off[g], e = off[g + 1];
+    Xyzz<O> acc;
+    if (s == e) {
+        xyzz_set_inf(acc);
+        st16(&buckets[g], acc);
+        return;
+    }
+    const uint32_t l0 = s / (uint32_t)K, l1 = (e - 1) / (uint32_t)K;
+    if (l0 == l1) return;
+    ld16(acc, &tail[l0]);
+    for (uint32_t l = l0 + 1; l <= l1; ++l) {
+        Xyzz<O> v;
+        ld16(v, &head[l]);
+        xyzz_add<O>(acc, acc, v);
     }
     st16(&buckets[g], acc);
 }

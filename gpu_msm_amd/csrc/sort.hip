@@ -11,7 +11,7 @@ hipError_t launch_sort(hipStream_t s, const Plan& p, const uint64_t* d_scalars, 
     uint32_t* cnt = reinterpret_cast<uint32_t*>(ws + p.o_cnt);
     uint32_t* off = reinterpret_cast<uint32_t*>(ws + p.o_off);
     uint32_t* rank = reinterpret_cast<uint32_t*>(ws + p.o_rank);
-    uint32_t* sorted = reinterpret_cast<uint32_t*>(ws + p.o_sorted);
+    uint64_t* sorted = reinterpret_cast<uint64_t*>(ws + p.o_sorted);
     uint32_t* err = reinterpret_cast<uint32_t*>(ws + p.o_err);
     const size_t nbk = (size_t)p.nw * p.B;
     auto mark = [&](int st) -> hipError_t {

@@ -83,7 +83,7 @@ __global__ void __launch_bounds__(256) k_digits_hist(const uint64_t* __restrict_
 
 __global__ void __launch_bounds__(256) k_scatter(const uint64_t* __restrict__ scalars, int n, int L, int c, int nw,
                           const uint32_t* __restrict__ off, const uint32_t* __restrict__ rank,
-                          uint32_t* __restrict__ sorted) {
+                          uint64_t* __restrict__ sorted) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     DigitIter it;
@@ -94,7 +94,8 @@ __global__ void __launch_bounds__(256) k_scatter(const uint64_t* __restrict__ sc
         if (d != 0) {
             const int b = (d < 0 ? -d : d) - 1;
             const uint32_t pos = off[(size_t)w * B + b] + rank[(size_t)w * n + i];
-            sorted[pos] = (uint32_t)i | (d < 0 ? 0x80000000u : 0u);
+            // (global bucket << 32) | point index | sign bit
+            sorted[pos] = ((uint64_t)((uint32_t)w * B + b) << 32) | (uint32_t)i | (d < 0 ? 0x80000000u : 0u);
         }
     }
 }

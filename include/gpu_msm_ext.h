@@ -70,6 +70,9 @@ int gm_msm_resident(void* result_affine, const gm_bases* b, int base_offset, con
 
 /* Record HIP events between pipeline stages of subsequent MSMs on this GPU. */
 int gm_set_profiling(uint32_t gpu_index, int on);
+/* Plan of the last MSM run on this GPU: window bits, windows, and the number
+ * of (window, bucket) entries the accumulation kernel added. */
+int gm_last_msm_info(uint32_t gpu_index, int* window, int* windows, long long* entries);
 /* Stage times (ms) of the last profiled MSM: [hist, scan, scatter,
  * accumulate, bucket_reduce, window_sum]. */
 int gm_profile_last(uint32_t gpu_index, float* stage_ms, int count);
