@@ -65,6 +65,19 @@ def closed_form_scalar(ks: np.ndarray, ss: np.ndarray) -> int:
     return acc % r
 
 
+def measured_traffic() -> float | None:
+    """HBM bytes per k_accumulate launch from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (profiles/k_accumulate_g1_2p20.md)."""
+    path = os.path.join(ROOT, "profiles", "k_accumulate_g1_2p20.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def time_msm(bases, scalars, steps: int, warmup: int, profile: bool):
     for _ in range(warmup):
         res = gm.msm_resident(bases, scalars, scalar_bits=299)
@@ -112,6 +125,8 @@ def main() -> None:
     ap.add_argument("--log2n", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-g2", action="store_true")
+    ap.add_argument("--no-self-check", action="store_true",
+                    help="skip the closed-form check (its n=1 MSM would mix into profiler averages)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU MSM is "
@@ -130,10 +145,12 @@ def main() -> None:
     info = gm.last_msm_info()
 
     # self-check by the closed form: MSM over k_i G equals t G, t computed on the host
-    t = closed_form_scalar(ks, ss)
-    g = gm.Bases.from_k(gm.CURVE_G1, np.array([1], dtype=np.uint64))
-    tg = gm.msm_resident(g, gm.Scalars.upload(gm.scalars_to_limbs([t])), scalar_bits=299)
-    self_check = res1.tobytes() == tg.tobytes()
+    self_check = None
+    if not args.no_self_check:
+        t = closed_form_scalar(ks, ss)
+        g = gm.Bases.from_k(gm.CURVE_G1, np.array([1], dtype=np.uint64))
+        tg = gm.msm_resident(g, gm.Scalars.upload(gm.scalars_to_limbs([t])), scalar_bits=299)
+        self_check = res1.tobytes() == tg.tobytes()
 
     acc_ms = float(np.mean([s["accumulate"] for s in stages]))
     achieved = info["entries"] * MAD_PER_MIXED_ADD / (acc_ms * 1e-3)
@@ -180,7 +197,7 @@ def main() -> None:
             "peak": round(VALU_PEAK_MAD_PER_S / 1e12, 3),
             "unit": "T v_mad_u64_u32/s (algorithmic: 11 M x 406 mad per mixed add)",
             "frac": round(achieved / VALU_PEAK_MAD_PER_S, 4),
-            "traffic": None,
+            "traffic": measured_traffic(),
             "hbm_algorithmic_GBs": round(acc_bytes / (acc_ms * 1e-3) / 1e9, 1),
             "hbm_peak_GBs": HBM_PEAK_GBS,
             "launch_ms": round(acc_ms, 3),

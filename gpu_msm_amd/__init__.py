@@ -50,7 +50,7 @@ EXPORTS = [
     "g1_is_infinity_wrapper", "g2_is_infinity_wrapper",
     "g1_msm_wrapper", "g2_msm_wrapper",
     "g1_from_montgomery_wrapper", "g2_from_montgomery_wrapper", "fp_to_montgomery_wrapper",
-    "g1_msm_multi_wrapper", "g2_msm_multi_wrapper", "gm_plan_query",
+    "init_device_generators", "g1_msm_multi_wrapper", "g2_msm_multi_wrapper", "gm_plan_query",
     "gm_bases_upload", "gm_bases_from_k", "gm_bases_download", "gm_bases_count", "gm_bases_free",
     "gm_scalars_upload", "gm_scalars_free", "gm_msm_resident",
     "gm_set_profiling", "gm_profile_last", "gm_last_msm_info",
@@ -89,6 +89,7 @@ def lib() -> ctypes.CDLL:
         "g1_from_montgomery_wrapper": ([vp, vp], None),
         "g2_from_montgomery_wrapper": ([vp, vp], None),
         "fp_to_montgomery_wrapper": ([vp, vp], None),
+        "init_device_generators": ([vp, u32], None),
         "g1_msm_multi_wrapper": ([vp, vp, vp, i, i, u32, u32], i),
         "g2_msm_multi_wrapper": ([vp, vp, vp, i, i, u32, u32], i),
         "gm_plan_query": ([i, i, i, i, i, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(sz)], i),
@@ -351,7 +352,7 @@ def set_profiling(on: bool, gpu_index: int = 0) -> None:
     _check(lib().gm_set_profiling(gpu_index, int(on)), "set profiling")
 
 
-STAGES = ["hist", "scan", "scatter", "accumulate", "bucket_reduce", "window_sum"]
+STAGES = ["hist", "scan", "scatter", "accumulate", "fixup", "bucket_reduce", "window_sum"]
 
 
 def profile_last(gpu_index: int = 0) -> dict:

@@ -1,4 +1,4 @@
-// Bucket accumulation kernel instantiation (g1).
+// Bucket accumulation and fix-up kernel instantiations (g1).
 #include "engine_impl.hpp"
 #include "msm_kernels.hpp"
 
@@ -16,10 +16,19 @@ hipError_t launch_accumulate(hipStream_t s, const Plan& p, const Aff<O>* d_bases
             d_bases, reinterpret_cast<const uint64_t*>(ws + p.o_sorted), off, nbk, p.chunk, buckets, head, tail);
         GM_TRY(hipGetLastError());
     }
-    k_bucket_fixup<O><<<blocks_for(nbk, 64), 64, 0, s>>>(off, nbk, p.chunk, head, tail, buckets);
+    return hipSuccess;
+}
+
+template <class O>
+hipError_t launch_fixup(hipStream_t s, const Plan& p, uint8_t* ws) {
+    const int nbk = p.nw * p.B;
+    k_bucket_fixup<O><<<blocks_for(nbk, 64), 64, 0, s>>>(
+        reinterpret_cast<const uint32_t*>(ws + p.o_off), nbk, p.chunk, reinterpret_cast<const Xyzz<O>*>(ws + p.o_head),
+        reinterpret_cast<const Xyzz<O>*>(ws + p.o_tail), reinterpret_cast<Xyzz<O>*>(ws + p.o_buckets));
     return hipGetLastError();
 }
 
 template hipError_t launch_accumulate<FpOps>(hipStream_t, const Plan&, const Aff<FpOps>*, uint8_t*);
+template hipError_t launch_fixup<FpOps>(hipStream_t, const Plan&, uint8_t*);
 
 }  // namespace gm

@@ -583,6 +583,12 @@ int g2_msm_wrapper(G2ProjectivePoint* result, const G2Point* points, const uint6
     return msm_host_ptrs<Fp2Ops>(result, points, scalars, 1, 64, n, gpu_index, false);
 }
 
+// reference rust-api/src/ffi.rs:67 (src/curve.cu:833-838)
+void init_device_generators(const void* stream, uint32_t gpu_index) {
+    (void)stream;
+    get_ctx(gpu_index);
+}
+
 // reference rust-api/src/c_wrapper.cu:261-283
 void g1_from_montgomery_wrapper(G1Point* result, const G1Point* point) {
     G1Point r;

@@ -30,7 +30,7 @@ struct AbiLayout<Fp2Ops> {
 size_t al_size(size_t x);
 
 // Profiling stages recorded with HIP events on the MSM's own stream.
-enum Stage { ST_START = 0, ST_HIST, ST_SCAN, ST_SCATTER, ST_ACCUM, ST_REDUCE, ST_WSUM, ST_COUNT };
+enum Stage { ST_START = 0, ST_HIST, ST_SCAN, ST_SCATTER, ST_ACCUM, ST_FIXUP, ST_REDUCE, ST_WSUM, ST_COUNT };
 
 struct Prof {
     hipEvent_t ev[ST_COUNT];
@@ -82,6 +82,8 @@ extern template struct Engine<Fp2Ops>;
 hipError_t launch_sort(hipStream_t s, const Plan& p, const uint64_t* d_scalars, uint8_t* ws, Prof* prof);
 template <class O>
 hipError_t launch_accumulate(hipStream_t s, const Plan& p, const Aff<O>* d_bases, uint8_t* ws);
+template <class O>
+hipError_t launch_fixup(hipStream_t s, const Plan& p, uint8_t* ws);
 template <class O>
 hipError_t launch_reduce(hipStream_t s, const Plan& p, uint8_t* ws);
 template <class O>

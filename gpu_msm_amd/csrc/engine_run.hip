@@ -13,6 +13,8 @@ hipError_t Engine<O>::run(hipStream_t s, const Plan& p, const Aff<O>* d_bases, c
     GM_TRY(launch_sort(s, p, d_scalars, ws, prof));
     GM_TRY(launch_accumulate<O>(s, p, d_bases, ws));
     GM_TRY(mark(ST_ACCUM));
+    GM_TRY(launch_fixup<O>(s, p, ws));
+    GM_TRY(mark(ST_FIXUP));
     GM_TRY(launch_reduce<O>(s, p, ws));
     GM_TRY(mark(ST_REDUCE));
     GM_TRY(launch_window_sum<O>(s, p, ws));

@@ -74,7 +74,7 @@ int gm_set_profiling(uint32_t gpu_index, int on);
  * of (window, bucket) entries the accumulation kernel added. */
 int gm_last_msm_info(uint32_t gpu_index, int* window, int* windows, long long* entries);
 /* Stage times (ms) of the last profiled MSM: [hist, scan, scatter,
- * accumulate, bucket_reduce, window_sum]. */
+ * accumulate, fixup, bucket_reduce, window_sum]. */
 int gm_profile_last(uint32_t gpu_index, float* stage_ms, int count);
 
 #ifdef __cplusplus

@@ -51,6 +51,12 @@ int g1_msm_wrapper(G1ProjectivePoint* result, const G1Point* points, const uint6
 int g2_msm_wrapper(G2ProjectivePoint* result, const G2Point* points, const uint64_t* scalars, int n,
                    uint32_t gpu_index);
 
+/* reference rust-api/src/ffi.rs:67 binds init_device_generators(stream,
+ * gpu_index) with C linkage (reference src/curve.cu:833-838). Generators are
+ * compile-time constants here, so this only selects and checks the device;
+ * `stream` is ignored and may be NULL. */
+void init_device_generators(const void* stream, uint32_t gpu_index);
+
 /* reference rust_wrapper.h:62-64 / c_wrapper.cu:261-288 */
 void g1_from_montgomery_wrapper(G1Point* result, const G1Point* point);
 void g2_from_montgomery_wrapper(G2Point* result, const G2Point* point);
