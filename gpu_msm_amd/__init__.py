@@ -52,7 +52,7 @@ EXPORTS = [
     "g1_from_montgomery_wrapper", "g2_from_montgomery_wrapper", "fp_to_montgomery_wrapper",
     "init_device_generators", "g1_msm_multi_wrapper", "g2_msm_multi_wrapper", "gm_plan_query",
     "gm_bases_upload", "gm_bases_from_k", "gm_bases_download", "gm_bases_count", "gm_bases_free",
-    "gm_scalars_upload", "gm_scalars_free", "gm_msm_resident",
+    "gm_scalars_upload", "gm_scalars_free", "gm_msm_resident", "gm_msm_batch",
     "gm_set_profiling", "gm_profile_last", "gm_last_msm_info",
 ]
 
@@ -101,6 +101,7 @@ def lib() -> ctypes.CDLL:
         "gm_scalars_upload": ([ctypes.POINTER(vp), vp, i, i, u32], i),
         "gm_scalars_free": ([vp], None),
         "gm_msm_resident": ([vp, vp, i, vp, i, i, i, i, i], i),
+        "gm_msm_batch": ([vp, vp, vp, vp, i, i, i], i),
         "gm_set_profiling": ([u32, i], i),
         "gm_profile_last": ([u32, ctypes.POINTER(ctypes.c_float), i], i),
         "gm_last_msm_info": ([u32, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(ctypes.c_longlong)], i),
@@ -338,6 +339,26 @@ def msm_resident(bases: Bases, scalars: Scalars, n: int | None = None, base_offs
     out = np.zeros(1, dtype=G1_DTYPE if bases.curve == CURVE_G1 else G2_DTYPE)
     _check(lib().gm_msm_resident(_ptr(out), bases.h, base_offset, scalars.h, scalar_offset, n, scalar_bits, window,
                                  int(device_combine)), "resident MSM")
+    return out
+
+
+SPEC_DTYPE = np.dtype([("n", "<i4"), ("base_offset", "<i4"), ("scalar_offset", "<i4"), ("window", "<i4"),
+                       ("seg", "<i4")])
+
+
+def msm_batch(bases: Bases, scalars: Scalars, specs, scalar_bits: int = 0, device_combine: bool = False) -> np.ndarray:
+    """Many MSMs in one pipeline run. specs: array of SPEC_DTYPE or sequence of
+    (n, base_offset, scalar_offset[, window[, seg]]). Returns affine ABI records."""
+    sp = np.zeros(len(specs), dtype=SPEC_DTYPE)
+    if isinstance(specs, np.ndarray) and specs.dtype == SPEC_DTYPE:
+        sp[:] = specs
+    else:
+        for k, s in enumerate(specs):
+            s = tuple(s) + (0,) * (5 - len(s))
+            sp[k] = s
+    out = np.zeros(len(sp), dtype=G1_DTYPE if bases.curve == CURVE_G1 else G2_DTYPE)
+    _check(lib().gm_msm_batch(_ptr(out), bases.h, scalars.h, _ptr(sp), len(sp), scalar_bits, int(device_combine)),
+           "batched MSM")
     return out
 
 

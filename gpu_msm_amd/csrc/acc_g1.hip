@@ -6,7 +6,7 @@ namespace gm {
 
 template <class O>
 hipError_t launch_accumulate(hipStream_t s, const Plan& p, const Aff<O>* d_bases, uint8_t* ws) {
-    const int nbk = p.nw * p.B;
+    const int nbk = (int)p.bkt_total;
     const uint32_t* off = reinterpret_cast<const uint32_t*>(ws + p.o_off);
     Xyzz<O>* buckets = reinterpret_cast<Xyzz<O>*>(ws + p.o_buckets);
     Xyzz<O>* head = reinterpret_cast<Xyzz<O>*>(ws + p.o_head);
@@ -21,7 +21,7 @@ hipError_t launch_accumulate(hipStream_t s, const Plan& p, const Aff<O>* d_bases
 
 template <class O>
 hipError_t launch_fixup(hipStream_t s, const Plan& p, uint8_t* ws) {
-    const int nbk = p.nw * p.B;
+    const int nbk = (int)p.bkt_total;
     k_bucket_fixup<O><<<blocks_for(nbk, 64), 64, 0, s>>>(
         reinterpret_cast<const uint32_t*>(ws + p.o_off), nbk, p.chunk, reinterpret_cast<const Xyzz<O>*>(ws + p.o_head),
         reinterpret_cast<const Xyzz<O>*>(ws + p.o_tail), reinterpret_cast<Xyzz<O>*>(ws + p.o_buckets));

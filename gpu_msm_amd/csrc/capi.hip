@@ -8,7 +8,10 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <mutex>
+#include <thread>
+#include <vector>
 
 #include "../../include/gpu_msm_ext.h"
 #include "../../include/rust_wrapper.h"
@@ -20,103 +23,6 @@ using gm::Fp2Ops;
 using gm::FpOps;
 using gm::Plan;
 using gm::Xyzz;
-
-namespace gm {
-
-size_t xyzz_bytes(int curve) { return curve == CURVE_G1 ? sizeof(Xyzz<FpOps>) : sizeof(Xyzz<Fp2Ops>); }
-size_t aff_bytes(int curve) { return curve == CURVE_G1 ? sizeof(Aff<FpOps>) : sizeof(Aff<Fp2Ops>); }
-
-static int windows_for(int bits, int c) { return (bits + 1 + c - 1) / c; }
-
-static int seg_for(int B) { return B < 32 ? B : 32; }
-
-// Relative cost in Fp-multiplication equivalents: mixed adds of the bucket
-// phase, running-sum adds and the per-segment scalar multiples of the bucket
-// reduction, and a per-(window, point) charge for recoding and sorting.
-static double plan_cost(int curve, int n, int bits, int c) {
-    const int nw = windows_for(bits, c);
-    const double B = (double)(1 << (c - 1));
-    const double seg = seg_for(1 << (c - 1));
-    const double fm = curve == CURVE_G1 ? 1.0 : 3.0;
-    const double accum = (double)nw * n * 10.5 * fm;
-    const double reduce = (double)nw * (B * 28.0 + (B / seg) * 18.0 * c) * fm;
-    const double sort = (double)nw * n * 0.6;
-    return accum + reduce + sort;
-}
-
-int choose_window(int curve, int n, int bits) {
-    int best = 2;
-    double bc = 1e300;
-    for (int c = 2; c <= 20; ++c) {
-        const double v = plan_cost(curve, n, bits, c);
-        if (v < bc) {
-            bc = v;
-            best = c;
-        }
-    }
-    return best;
-}
-
-size_t al_size(size_t x) { return (x + 255) & ~(size_t)255; }
-static size_t al(size_t x) { return al_size(x); }
-
-bool make_plan(Plan& p, int curve, int n, int L, int bits, int c) {
-    memset(&p, 0, sizeof p);
-    if (n < 0 || L < 1 || L > 8) return false;
-    if (bits <= 0 || bits > 64 * L) bits = 64 * L;
-    if (c <= 0) c = choose_window(curve, n, bits);
-    if (c < 2 || c > 24) return false;
-    p.curve = curve;
-    p.n = n;
-    p.L = L;
-    p.bits = bits;
-    p.c = c;
-    p.nw = windows_for(bits, c);
-    p.B = 1 << (c - 1);
-    p.seg = seg_for(p.B);
-    p.nseg = p.B / p.seg;
-    p.chunk = 32;
-    p.lanes = (int)(((size_t)p.nw * n + p.chunk - 1) / p.chunk);
-    const size_t nbk = (size_t)p.nw * p.B;
-    size_t cub = 0;
-    if (hipcub::DeviceScan::ExclusiveSum(nullptr, cub, (uint32_t*)nullptr, (uint32_t*)nullptr, (int)(nbk + 1)) !=
-        hipSuccess) {
-        // No device to ask (plan queries on a host without a GPU): a bound on
-        // the decoupled-lookback scan state.
-        cub = (nbk / 128 + 1024) * 16 + 65536;
-    }
-    p.cub_bytes = cub;
-    const size_t xb = xyzz_bytes(curve);
-    size_t o = 0;
-    p.o_cnt = o;
-    o += al((nbk + 1) * 4);
-    p.o_off = o;
-    o += al((nbk + 1) * 4);
-    p.o_rank = o;
-    o += al((size_t)p.nw * n * 4 + 4);
-    p.o_sorted = o;
-    o += al((size_t)p.nw * n * 8 + 8);
-    p.o_buckets = o;
-    o += al(nbk * xb);
-    p.o_head = o;
-    o += al((size_t)p.lanes * xb);
-    p.o_tail = o;
-    o += al((size_t)p.lanes * xb);
-    p.o_segs = o;
-    o += al((size_t)p.nw * p.nseg * xb);
-    p.o_wsum = o;
-    o += al((size_t)p.nw * xb);
-    p.o_out = o;
-    o += al(aff_bytes(curve));
-    p.o_err = o;
-    o += al(4);
-    p.o_cub = o;
-    o += al(cub);
-    p.total = o;
-    return true;
-}
-
-}  // namespace gm
 
 // ------------------------------------------------------------------ context
 
@@ -288,55 +194,84 @@ struct HostOf<Fp2Ops> {
     static constexpr int CURVE = gm::CURVE_G2;
 };
 
-// Runs the device pipeline for an already-resident set of bases/scalars and
-// finishes on the host (or device). Caller holds c->mu. Writes affine x, y.
+template <class HF>
+struct AffResult {
+    bool finite;
+    HF x, y;
+};
+
+// Runs the pipeline for a batch of MSMs over resident bases/scalars and
+// finishes each with a host (or device) window combine. Caller holds c->mu.
 template <class O>
-int run_msm_locked(DevCtx* c, const Aff<O>* d_bases, const uint64_t* d_scalars, int n, int L, int bits, int window,
-                   bool device_combine, typename HostOf<O>::HF& x, typename HostOf<O>::HF& y, bool& finite,
-                   uint8_t* ws_override = nullptr) {
+int run_batch_locked(DevCtx* c, const Aff<O>* d_bases, const uint64_t* d_scalars, int L, int bits,
+                     const gm::MsmSpec* specs, int count, bool device_combine,
+                     std::vector<AffResult<typename HostOf<O>::HF>>& out, uint8_t* ws_override = nullptr) {
     using HF = typename HostOf<O>::HF;
+    std::vector<gm::MsmDesc> desc((size_t)count);
     Plan p;
-    if (!gm::make_plan(p, HostOf<O>::CURVE, n, L, bits, window)) return -1;
+    if (!gm::make_plan(p, desc.data(), HostOf<O>::CURVE, L, bits, specs, count)) return -1;
     uint8_t* ws = ws_override;
     if (!ws) {
         if (ensure_ws(c, p.total) != hipSuccess) return -3;
         ws = c->ws;
     }
-    const size_t wsb = (size_t)p.nw * sizeof(Xyzz<O>);
-    if (ensure_pinned(c, wsb + sizeof(Aff<O>) + 256) != hipSuccess) return -3;
+    const size_t desc_b = (size_t)count * sizeof(gm::MsmDesc);
+    const size_t res_b = device_combine ? (size_t)count * sizeof(Aff<O>) : (size_t)p.win_total * sizeof(Xyzz<O>);
+    const size_t o_res = gm::al_size(desc_b), o_flags = o_res + gm::al_size(res_b);
+    if (ensure_pinned(c, o_flags + 256) != hipSuccess) return -3;
+    uint8_t* h = c->pinned;
+    memcpy(h, desc.data(), desc_b);
+    if (hipMemcpyAsync(ws + p.o_desc, h, desc_b, hipMemcpyHostToDevice, c->stream) != hipSuccess) return -4;
     if (gm::Engine<O>::run(c->stream, p, d_bases, d_scalars, ws, &c->prof) != hipSuccess) return -4;
     c->prof_valid = c->prof.on;
-    uint8_t* h = c->pinned;
-    if (hipMemcpyAsync(h + wsb + sizeof(Aff<O>), ws + p.o_err, 4, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
-        return -4;
-    if (hipMemcpyAsync(h + wsb + sizeof(Aff<O>) + 4, ws + p.o_off + (size_t)p.nw * p.B * 4, 4, hipMemcpyDeviceToHost,
+    if (hipMemcpyAsync(h + o_flags, ws + p.o_err, 4, hipMemcpyDeviceToHost, c->stream) != hipSuccess) return -4;
+    if (hipMemcpyAsync(h + o_flags + 4, ws + p.o_off + (size_t)p.bkt_total * 4, 4, hipMemcpyDeviceToHost,
                        c->stream) != hipSuccess)
         return -4;
     if (device_combine) {
         if (gm::Engine<O>::device_combine(c->stream, p, ws) != hipSuccess) return -4;
-        if (hipMemcpyAsync(h + wsb, ws + p.o_out, sizeof(Aff<O>), hipMemcpyDeviceToHost, c->stream) != hipSuccess)
-            return -4;
+        if (hipMemcpyAsync(h + o_res, ws + p.o_out, res_b, hipMemcpyDeviceToHost, c->stream) != hipSuccess) return -4;
     } else {
-        if (hipMemcpyAsync(h, ws + p.o_wsum, wsb, hipMemcpyDeviceToHost, c->stream) != hipSuccess) return -4;
+        if (hipMemcpyAsync(h + o_res, ws + p.o_wsum, res_b, hipMemcpyDeviceToHost, c->stream) != hipSuccess) return -4;
     }
     if (hipStreamSynchronize(c->stream) != hipSuccess) return -4;
     uint32_t err, entries;
-    memcpy(&err, h + wsb + sizeof(Aff<O>), 4);
-    memcpy(&entries, h + wsb + sizeof(Aff<O>) + 4, 4);
+    memcpy(&err, h + o_flags, 4);
+    memcpy(&entries, h + o_flags + 4, 4);
     if (err) return -5;
-    c->last_c = p.c;
-    c->last_nw = p.nw;
+    c->last_c = (int)desc[0].c;
+    c->last_nw = (int)desc[0].nw;
     c->last_entries = entries;
+    out.resize((size_t)count);
+    constexpr int W = words_of<HF>();
     if (device_combine) {
-        const uint32_t* a = reinterpret_cast<const uint32_t*>(h + wsb);
-        constexpr int W = words_of<HF>();
-        finite = a[15] != gm::INF_MARK;
-        if (finite) {
-            dev_elem_to_host(x, a);
-            dev_elem_to_host(y, a + W);
+        for (int j = 0; j < count; ++j) {
+            const uint32_t* a = reinterpret_cast<const uint32_t*>(h + o_res + (size_t)j * sizeof(Aff<O>));
+            out[j].finite = a[15] != gm::INF_MARK;
+            if (out[j].finite) {
+                dev_elem_to_host(out[j].x, a);
+                dev_elem_to_host(out[j].y, a + W);
+            }
         }
     } else {
-        finite = host_combine<HF>(x, y, reinterpret_cast<const uint32_t*>(h), p.nw, p.c);
+        const uint32_t* raw = reinterpret_cast<const uint32_t*>(h + o_res);
+        auto work = [&](int j0, int j1) {
+            for (int j = j0; j < j1; ++j)
+                out[j].finite = host_combine<HF>(out[j].x, out[j].y, raw + (size_t)desc[j].win_base * 4 * W,
+                                                 (int)desc[j].nw, (int)desc[j].c);
+        };
+        const int nt = count < 8 ? 1 : (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
+        if (nt == 1) {
+            work(0, count);
+        } else {
+            std::vector<std::thread> th;
+            const int per = (count + nt - 1) / nt;
+            for (int t = 0; t < nt; ++t) {
+                const int j0 = t * per, j1 = std::min(count, j0 + per);
+                if (j0 < j1) th.emplace_back(work, j0, j1);
+            }
+            for (auto& t : th) t.join();
+        }
     }
     return 0;
 }
@@ -353,8 +288,10 @@ int msm_host_ptrs(void* result_proj, const void* points, const uint64_t* scalars
     if (!c) return -2;
     std::lock_guard<std::mutex> lk(c->mu);
     if (hipSetDevice(c->dev) != hipSuccess) return -2;
+    gm::MsmSpec spec = {n, 0, 0, 0, 0};
+    gm::MsmDesc d1;
     Plan p;
-    if (!gm::make_plan(p, HostOf<O>::CURVE, n, L, bits, 0)) return -1;
+    if (!gm::make_plan(p, &d1, HostOf<O>::CURVE, L, bits, &spec, 1)) return -1;
     const size_t b_abi = gm::al_size((size_t)n * Lay::AFF_BYTES);
     const size_t b_aff = gm::al_size((size_t)n * sizeof(Aff<O>));
     const size_t b_sc = gm::al_size((size_t)n * L * 8);
@@ -367,11 +304,10 @@ int msm_host_ptrs(void* result_proj, const void* points, const uint64_t* scalars
         return -4;
     if (hipMemcpyAsync(d_sc, scalars, (size_t)n * L * 8, hipMemcpyHostToDevice, c->stream) != hipSuccess) return -4;
     if (gm::Engine<O>::import_aff(c->stream, d_aff, d_abi, n, to_mont) != hipSuccess) return -4;
-    HF x, y;
-    bool finite = false;
-    int rc = run_msm_locked<O>(c, d_aff, d_sc, n, L, bits, 0, false, x, y, finite, ws);
+    std::vector<AffResult<HF>> res;
+    int rc = run_batch_locked<O>(c, d_aff, d_sc, L, bits, &spec, 1, false, res, ws);
     if (rc) return rc;
-    write_proj_abi<HF>(result_proj, finite, x, y);
+    write_proj_abi<HF>(result_proj, res[0].finite, res[0].x, res[0].y);
     return 0;
 }
 
@@ -478,22 +414,29 @@ int bases_create(gm_bases** out, const void* host_points, const uint64_t* host_k
 }
 
 template <class O>
-int resident_msm(void* result_affine, const gm_bases* b, int base_offset, const gm_scalars* s, int scalar_offset,
-                 int n, int bits, int window, int device_combine) {
+int resident_batch(void* results_affine, const gm_bases* b, const gm_scalars* s, const gm_msm_spec* specs, int count,
+                   int bits, int device_combine) {
     using HF = typename HostOf<O>::HF;
-    if (!result_affine || !b || !s || n <= 0 || base_offset < 0 || scalar_offset < 0) return -1;
-    if ((long)base_offset + n > b->n || (long)scalar_offset + n > s->n || b->dev != s->dev) return -1;
+    if (!results_affine || !b || !s || !specs || count <= 0 || b->dev != s->dev) return -1;
+    std::vector<gm::MsmSpec> sp((size_t)count);
+    for (int j = 0; j < count; ++j) {
+        const gm_msm_spec& q = specs[j];
+        if (q.n <= 0 || q.base_offset < 0 || q.scalar_offset < 0) return -1;
+        if ((long)q.base_offset + q.n > b->n || (long)q.scalar_offset + q.n > s->n) return -1;
+        sp[j] = {q.n, q.base_offset, q.scalar_offset, q.window, q.seg};
+    }
     DevCtx* c = get_ctx((uint32_t)b->dev);
     if (!c) return -2;
     std::lock_guard<std::mutex> lk(c->mu);
     if (hipSetDevice(c->dev) != hipSuccess) return -2;
-    HF x, y;
-    bool finite = false;
-    const Aff<O>* bases = reinterpret_cast<const Aff<O>*>(b->d) + base_offset;
-    const uint64_t* sc = s->d + (size_t)scalar_offset * s->limbs;
-    int rc = run_msm_locked<O>(c, bases, sc, n, s->limbs, bits, window, device_combine != 0, x, y, finite);
+    std::vector<AffResult<HF>> res;
+    int rc = run_batch_locked<O>(c, reinterpret_cast<const Aff<O>*>(b->d), s->d, s->limbs, bits, sp.data(), count,
+                                 device_combine != 0, res);
     if (rc) return rc;
-    write_affine_abi<HF>(result_affine, finite, x, y);
+    const size_t rec = HostOf<O>::CURVE == gm::CURVE_G1 ? 120 : 232;
+    for (int j = 0; j < count; ++j)
+        write_affine_abi<HF>(reinterpret_cast<uint8_t*>(results_affine) + (size_t)j * rec, res[j].finite, res[j].x,
+                             res[j].y);
     return 0;
 }
 
@@ -649,10 +592,12 @@ int g2_msm_multi_wrapper(G2ProjectivePoint* result, const G2Point* points, const
 int gm_plan_query(int curve, int n, int scalar_limbs, int scalar_bits, int window, int* c_out, int* windows_out,
                   size_t* workspace_bytes_out) {
     if (curve != GM_CURVE_G1 && curve != GM_CURVE_G2) return -1;
+    gm::MsmSpec spec = {n, 0, 0, window, 0};
+    gm::MsmDesc d;
     Plan p;
-    if (!gm::make_plan(p, curve, n, scalar_limbs, scalar_bits, window)) return -1;
-    if (c_out) *c_out = p.c;
-    if (windows_out) *windows_out = p.nw;
+    if (!gm::make_plan(p, &d, curve, scalar_limbs, scalar_bits, &spec, 1)) return -1;
+    if (c_out) *c_out = (int)d.c;
+    if (windows_out) *windows_out = (int)d.nw;
     if (workspace_bytes_out) *workspace_bytes_out = p.total;
     return 0;
 }
@@ -726,12 +671,16 @@ void gm_scalars_free(gm_scalars* s) {
 
 int gm_msm_resident(void* result_affine, const gm_bases* b, int base_offset, const gm_scalars* s, int scalar_offset,
                     int n, int scalar_bits, int window, int device_combine) {
+    gm_msm_spec spec = {n, base_offset, scalar_offset, window, 0};
+    return gm_msm_batch(result_affine, b, s, &spec, 1, scalar_bits, device_combine);
+}
+
+int gm_msm_batch(void* results_affine, const gm_bases* b, const gm_scalars* s, const gm_msm_spec* specs, int count,
+                 int scalar_bits, int device_combine) {
     if (!b) return -1;
     if (b->curve == GM_CURVE_G1)
-        return resident_msm<FpOps>(result_affine, b, base_offset, s, scalar_offset, n, scalar_bits, window,
-                                   device_combine);
-    return resident_msm<Fp2Ops>(result_affine, b, base_offset, s, scalar_offset, n, scalar_bits, window,
-                                device_combine);
+        return resident_batch<FpOps>(results_affine, b, s, specs, count, scalar_bits, device_combine);
+    return resident_batch<Fp2Ops>(results_affine, b, s, specs, count, scalar_bits, device_combine);
 }
 
 int gm_set_profiling(uint32_t gpu_index, int on) {

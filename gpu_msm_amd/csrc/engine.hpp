@@ -37,36 +37,65 @@ struct Prof {
     bool on;
 };
 
+// One MSM of a batch, as the kernels see it (64 B, device resident).
+struct MsmDesc {
+    uint32_t n;          // points
+    uint32_t base_off;   // first base in the resident pool
+    uint32_t scal_off;   // first scalar (in scalars, each `L` limbs)
+    uint32_t c;          // window bits
+    uint32_t nw;         // windows
+    uint32_t B;          // buckets per window = 2^(c-1)
+    uint32_t seg;        // buckets per bucket-reduce lane
+    uint32_t nseg;       // segments per window = B / seg
+    uint32_t bkt_base;   // first global bucket
+    uint32_t pt_base;    // first flattened point
+    uint32_t rank_base;  // first (window, point) rank slot
+    uint32_t seg_base;   // first global segment
+    uint32_t win_base;   // first global window
+    uint32_t pad[3];
+};
+static_assert(sizeof(MsmDesc) == 64, "MsmDesc layout");
+
+// What a caller asks for: c <= 0 and seg <= 0 pick the defaults.
+struct MsmSpec {
+    int n;
+    int base_off;
+    int scal_off;
+    int c;
+    int seg;
+};
+
 struct Plan {
     int curve;
-    int n;        // points
-    int L;        // u64 limbs per scalar
-    int bits;     // scalar bit bound (values must be < 2^bits)
-    int c;        // window bits
-    int nw;       // windows
-    int B;        // buckets per window = 2^(c-1)
-    int seg;      // buckets per bucket-reduce lane
-    int nseg;     // segments per window
-    int chunk;    // entries per accumulation lane
-    int lanes;    // accumulation lanes (upper bound: all digits non-zero)
+    int L;          // u64 limbs per scalar (all MSMs of a batch)
+    int bits;       // scalar bit bound (values must be < 2^bits)
+    int count;      // MSMs in the batch
+    uint32_t n_total, ent_total, bkt_total, seg_total, win_total;
+    int chunk;      // entries per accumulation lane
+    int lanes;      // accumulation lanes (upper bound: all digits non-zero)
     size_t cub_bytes;
     // workspace offsets (bytes)
-    size_t o_cnt, o_off, o_rank, o_sorted, o_buckets, o_head, o_tail, o_segs, o_wsum, o_out, o_err, o_cub, total;
+    size_t o_desc, o_cnt, o_off, o_rank, o_sorted, o_buckets, o_head, o_tail, o_segs, o_wsum, o_out, o_err, o_cub,
+        total;
+    MsmDesc* host_desc;  // owned by the caller of make_plan (count entries)
 };
 
 size_t xyzz_bytes(int curve);
 size_t aff_bytes(int curve);
 int choose_window(int curve, int n, int bits);
-// c <= 0 picks the analytic heuristic.
-bool make_plan(Plan& p, int curve, int n, int L, int bits, int c);
+int choose_seg(int B);
+// Fills p (and p.host_desc[0..count)) for a batch; false on invalid specs.
+bool make_plan(Plan& p, MsmDesc* desc_storage, int curve, int L, int bits, const MsmSpec* specs, int count);
 
 template <class O>
 struct Engine {
     // Runs digits -> sort -> accumulate -> reduce -> window sums into the
-    // workspace (window sums at p.o_wsum). Asynchronous on `s`.
+    // workspace (window sums at p.o_wsum, MSM j's at win_base_j). The
+    // descriptors must already be at p.o_desc. Asynchronous on `s`.
     static hipError_t run(hipStream_t s, const Plan& p, const Aff<O>* d_bases, const uint64_t* d_scalars, uint8_t* d_ws,
                           Prof* prof);
-    // Horner over window sums on the device, canonical affine at p.o_out.
+    // Horner over window sums on the device, one lane per MSM; canonical
+    // affine results at p.o_out (count entries).
     static hipError_t device_combine(hipStream_t s, const Plan& p, uint8_t* d_ws);
     static hipError_t import_aff(hipStream_t s, Aff<O>* out, const void* d_abi, int n, bool to_mont);
     static hipError_t export_aff(hipStream_t s, void* d_abi, const Aff<O>* in, int n);
@@ -89,7 +118,7 @@ hipError_t launch_reduce(hipStream_t s, const Plan& p, uint8_t* ws);
 template <class O>
 hipError_t launch_window_sum(hipStream_t s, const Plan& p, uint8_t* ws);
 template <class O>
-hipError_t launch_horner(hipStream_t s, const Xyzz<O>* wsum, int count, int nw, int c, Aff<O>* out);
+hipError_t launch_horner(hipStream_t s, const MsmDesc* desc, int count, const Xyzz<O>* wsum, Aff<O>* out);
 template <class O>
 hipError_t launch_import(hipStream_t s, Aff<O>* out, const void* d_abi, int n, bool to_mont);
 template <class O>
@@ -98,5 +127,6 @@ template <class O>
 hipError_t launch_gen_bases(hipStream_t s, Aff<O>* out, const uint64_t* d_k, int n, const Aff<O>& g);
 template <class O>
 hipError_t launch_aff_to_abi_proj(hipStream_t s, void* d_abi_proj, const Aff<O>* in, int count);
+hipError_t launch_fill_u32(hipStream_t s, uint32_t* p, size_t n, uint32_t v);
 
 }  // namespace gm

@@ -24,8 +24,8 @@ hipError_t Engine<O>::run(hipStream_t s, const Plan& p, const Aff<O>* d_bases, c
 
 template <class O>
 hipError_t Engine<O>::device_combine(hipStream_t s, const Plan& p, uint8_t* ws) {
-    return launch_horner<O>(s, reinterpret_cast<const Xyzz<O>*>(ws + p.o_wsum), 1, p.nw, p.c,
-                            reinterpret_cast<Aff<O>*>(ws + p.o_out));
+    return launch_horner<O>(s, reinterpret_cast<const MsmDesc*>(ws + p.o_desc), p.count,
+                            reinterpret_cast<const Xyzz<O>*>(ws + p.o_wsum), reinterpret_cast<Aff<O>*>(ws + p.o_out));
 }
 
 template <class O>

@@ -5,8 +5,9 @@
 namespace gm {
 
 template <class O>
-hipError_t launch_horner(hipStream_t s, const Xyzz<O>* wsum, int count, int nw, int c, Aff<O>* out) {
-    k_horner<O><<<blocks_for(count, 64), 64, 0, s>>>(wsum, count, nw, c, out);
+hipError_t launch_horner(hipStream_t s, const MsmDesc* desc, int count, const Xyzz<O>* wsum, Aff<O>* out) {
+    if (count <= 0) return hipSuccess;
+    k_horner<O><<<blocks_for(count, 64), 64, 0, s>>>(desc, count, wsum, out);
     return hipGetLastError();
 }
 
@@ -37,7 +38,7 @@ hipError_t launch_aff_to_abi_proj(hipStream_t s, void* d_abi_proj, const Aff<O>*
     return hipGetLastError();
 }
 
-template hipError_t launch_horner<FpOps>(hipStream_t, const Xyzz<FpOps>*, int, int, int, Aff<FpOps>*);
+template hipError_t launch_horner<FpOps>(hipStream_t, const MsmDesc*, int, const Xyzz<FpOps>*, Aff<FpOps>*);
 template hipError_t launch_import<FpOps>(hipStream_t, Aff<FpOps>*, const void*, int, bool);
 template hipError_t launch_export<FpOps>(hipStream_t, void*, const Aff<FpOps>*, int);
 template hipError_t launch_gen_bases<FpOps>(hipStream_t, Aff<FpOps>*, const uint64_t*, int, const Aff<FpOps>&);

@@ -5,8 +5,9 @@
 namespace gm {
 
 template <class O>
-hipError_t launch_horner(hipStream_t s, const Xyzz<O>* wsum, int count, int nw, int c, Aff<O>* out) {
-    k_horner<O><<<blocks_for(count, 64), 64, 0, s>>>(wsum, count, nw, c, out);
+hipError_t launch_horner(hipStream_t s, const MsmDesc* desc, int count, const Xyzz<O>* wsum, Aff<O>* out) {
+    if (count <= 0) return hipSuccess;
+    k_horner<O><<<blocks_for(count, 64), 64, 0, s>>>(desc, count, wsum, out);
     return hipGetLastError();
 }
 
@@ -37,7 +38,7 @@ hipError_t launch_aff_to_abi_proj(hipStream_t s, void* d_abi_proj, const Aff<O>*
     return hipGetLastError();
 }
 
-template hipError_t launch_horner<Fp2Ops>(hipStream_t, const Xyzz<Fp2Ops>*, int, int, int, Aff<Fp2Ops>*);
+template hipError_t launch_horner<Fp2Ops>(hipStream_t, const MsmDesc*, int, const Xyzz<Fp2Ops>*, Aff<Fp2Ops>*);
 template hipError_t launch_import<Fp2Ops>(hipStream_t, Aff<Fp2Ops>*, const void*, int, bool);
 template hipError_t launch_export<Fp2Ops>(hipStream_t, void*, const Aff<Fp2Ops>*, int);
 template hipError_t launch_gen_bases<Fp2Ops>(hipStream_t, Aff<Fp2Ops>*, const uint64_t*, int, const Aff<Fp2Ops>&);

@@ -22,6 +22,7 @@
 //                  window; here either on the host once, or on the device)
 #pragma once
 #include "curve.hpp"
+#include "desc_find.hpp"
 
 namespace gm {
 
@@ -243,26 +244,29 @@ __global__ void __launch_bounds__(64) k_bucket_fixup(const uint32_t* __restrict_
 
 // ----------------------------------------------------------- bucket reduce
 
-// One lane per (window, segment of seg buckets). Bucket b of a window holds
-// digit magnitude b+1. Output: sum_{b in seg} (b+1) B_b.
+// One lane per (MSM, window, segment of seg buckets). Bucket b of a window
+// holds digit magnitude b+1. Output: sum_{b in segment} (b+1) B_b, computed
+// as T + lo * S from running sums S = sum B_b, T = sum (b - lo + 1) B_b.
 template <class O>
-__global__ void __launch_bounds__(64) k_bucket_reduce(const Xyzz<O>* __restrict__ buckets, int nw, int B, int seg,
+__global__ void __launch_bounds__(64) k_bucket_reduce(const MsmDesc* __restrict__ desc, int count, uint32_t seg_total,
+                                                      const Xyzz<O>* __restrict__ buckets,
                                                       Xyzz<O>* __restrict__ segs) {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    const int nseg = B / seg;
-    if (g >= nw * nseg) return;
-    const int w = g / nseg, t = g - w * nseg;
-    const Xyzz<O>* bw = buckets + (size_t)w * B + (size_t)t * seg;
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= seg_total) return;
+    const MsmDesc& m = desc[GM_FIND(desc, count, seg_base, g)];
+    const uint32_t local = g - m.seg_base;
+    const uint32_t w = local / m.nseg, t = local - w * m.nseg;
+    const Xyzz<O>* bw = buckets + m.bkt_base + (size_t)w * m.B + (size_t)t * m.seg;
     Xyzz<O> S, T;
     xyzz_set_inf(S);
     xyzz_set_inf(T);
-    for (int b = seg - 1; b >= 0; --b) {
+    for (int b = (int)m.seg - 1; b >= 0; --b) {
         Xyzz<O> bk;
         ld16(bk, &bw[b]);
         xyzz_add<O>(S, S, bk);
         xyzz_add<O>(T, T, S);
     }
-    const uint32_t lo = (uint32_t)t * seg;
+    const uint32_t lo = t * m.seg;
     if (lo != 0 && !xyzz_is_inf(S)) {
         Xyzz<O> M;
         xyzz_set_inf(M);
@@ -276,17 +280,22 @@ __global__ void __launch_bounds__(64) k_bucket_reduce(const Xyzz<O>* __restrict_
     st16(&segs[g], T);
 }
 
-// One workgroup per window: tree sum of nseg segment results.
+// One workgroup per (MSM, window): tree sum of its segment results.
 template <class O, int THREADS>
-__global__ void __launch_bounds__(THREADS) k_window_sum(const Xyzz<O>* __restrict__ segs, int nseg,
+__global__ void __launch_bounds__(THREADS) k_window_sum(const MsmDesc* __restrict__ desc, int count,
+                                                        const Xyzz<O>* __restrict__ segs,
                                                         Xyzz<O>* __restrict__ wsum) {
     __shared__ Xyzz<O> sh[THREADS];
-    const int w = blockIdx.x, tid = threadIdx.x;
+    const uint32_t gw = blockIdx.x;
+    const int tid = threadIdx.x;
+    const MsmDesc& m = desc[GM_FIND(desc, count, win_base, gw)];
+    const uint32_t w = gw - m.win_base;
+    const Xyzz<O>* sw = segs + m.seg_base + (size_t)w * m.nseg;
     Xyzz<O> acc;
     xyzz_set_inf(acc);
-    for (int t = tid; t < nseg; t += THREADS) {
+    for (uint32_t t = tid; t < m.nseg; t += THREADS) {
         Xyzz<O> v;
-        ld16(v, &segs[(size_t)w * nseg + t]);
+        ld16(v, &sw[t]);
         xyzz_add<O>(acc, acc, v);
     }
     sh[tid] = acc;
@@ -299,28 +308,29 @@ __global__ void __launch_bounds__(THREADS) k_window_sum(const Xyzz<O>* __restric
         }
         __syncthreads();
     }
-    if (tid == 0) st16(&wsum[w], sh[0]);
+    if (tid == 0) st16(&wsum[gw], sh[0]);
 }
 
-// Horner over window sums on the device, one lane per MSM (window sums of MSM
-// m at wsum[m * nw ...]); writes the canonical affine result of each MSM.
+// Horner over window sums on the device, one lane per MSM; writes the
+// canonical affine result of each MSM.
 template <class O>
-__global__ void __launch_bounds__(64) k_horner(const Xyzz<O>* __restrict__ wsum, int count, int nw, int c,
-                                               Aff<O>* __restrict__ out) {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= count) return;
-    const Xyzz<O>* ws = wsum + (size_t)m * nw;
+__global__ void __launch_bounds__(64) k_horner(const MsmDesc* __restrict__ desc, int count,
+                                               const Xyzz<O>* __restrict__ wsum, Aff<O>* __restrict__ out) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= count) return;
+    const MsmDesc& m = desc[j];
+    const Xyzz<O>* ws = wsum + m.win_base;
     Xyzz<O> r;
-    ld16(r, &ws[nw - 1]);
-    for (int w = nw - 2; w >= 0; --w) {
-        for (int k = 0; k < c; ++k) xyzz_dbl<O>(r, r);
+    ld16(r, &ws[m.nw - 1]);
+    for (int w = (int)m.nw - 2; w >= 0; --w) {
+        for (uint32_t k = 0; k < m.c; ++k) xyzz_dbl<O>(r, r);
         Xyzz<O> v;
         ld16(v, &ws[w]);
         xyzz_add<O>(r, r, v);
     }
     Aff<O> a;
     if (!xyzz_to_aff<O>(a.x, a.y, r)) aff_set_inf(a);
-    st16(&out[m], a);
+    st16(&out[j], a);
 }
 
 // device affine -> ABI projective (X, Y, Z) = (x, y, R mod p) or (0, 0, 0),

@@ -1,0 +1,131 @@
+// Batch plans: per-MSM window choice, descriptor prefix sums and the
+// workspace layout of one pipeline run.
+#include <hipcub/hipcub.hpp>
+
+#include <string.h>
+
+#include "engine.hpp"
+
+namespace gm {
+
+size_t xyzz_bytes(int curve) { return curve == CURVE_G1 ? sizeof(Xyzz<FpOps>) : sizeof(Xyzz<Fp2Ops>); }
+size_t aff_bytes(int curve) { return curve == CURVE_G1 ? sizeof(Aff<FpOps>) : sizeof(Aff<Fp2Ops>); }
+size_t al_size(size_t x) { return (x + 255) & ~(size_t)255; }
+
+static int windows_for(int bits, int c) { return (bits + 1 + c - 1) / c; }
+
+// Buckets per bucket-reduce lane: short segments keep enough lanes in flight
+// (the reduction is latency-bound); the price is one small scalar multiple
+// per segment.
+int choose_seg(int B) { return B < 8 ? B : 8; }
+
+// Relative cost in Fp-multiplication equivalents: mixed adds of the bucket
+// phase (10 products each, 3x for Fp2), the running-sum adds and per-segment
+// scalar multiples of the bucket reduction, and a per-(window, point) charge
+// for recoding and sorting.
+static double plan_cost(int curve, int n, int bits, int c) {
+    const int nw = windows_for(bits, c);
+    const double B = (double)(1 << (c - 1));
+    const double seg = choose_seg(1 << (c - 1));
+    const double fm = curve == CURVE_G1 ? 1.0 : 3.0;
+    const double accum = (double)nw * n * 10.5 * fm;
+    const double reduce = (double)nw * (B * 28.0 + (B / seg) * 18.0 * c) * fm;
+    const double sort = (double)nw * n * 0.6;
+    return accum + reduce + sort;
+}
+
+int choose_window(int curve, int n, int bits) {
+    int best = 2;
+    double bc = 1e300;
+    for (int c = 2; c <= 20; ++c) {
+        const double v = plan_cost(curve, n, bits, c);
+        if (v < bc) {
+            bc = v;
+            best = c;
+        }
+    }
+    return best;
+}
+
+bool make_plan(Plan& p, MsmDesc* d, int curve, int L, int bits, const MsmSpec* specs, int count) {
+    memset(&p, 0, sizeof p);
+    if (count < 1 || L < 1 || L > 8 || !d || !specs) return false;
+    if (bits <= 0 || bits > 64 * L) bits = 64 * L;
+    p.curve = curve;
+    p.L = L;
+    p.bits = bits;
+    p.count = count;
+    p.host_desc = d;
+    uint64_t pt = 0, rank = 0, bkt = 0, seg = 0, win = 0;
+    for (int j = 0; j < count; ++j) {
+        const MsmSpec& s = specs[j];
+        if (s.n < 0 || s.base_off < 0 || s.scal_off < 0) return false;
+        const int c = s.c > 0 ? s.c : choose_window(curve, s.n, bits);
+        if (c < 2 || c > 24) return false;
+        const int nw = windows_for(bits, c);
+        const int B = 1 << (c - 1);
+        int sg = s.seg > 0 ? s.seg : choose_seg(B);
+        if (sg > B) sg = B;
+        if ((sg & (sg - 1)) != 0) return false;  // power of two dividing B
+        MsmDesc& m = d[j];
+        memset(&m, 0, sizeof m);
+        m.n = (uint32_t)s.n;
+        m.base_off = (uint32_t)s.base_off;
+        m.scal_off = (uint32_t)s.scal_off;
+        m.c = (uint32_t)c;
+        m.nw = (uint32_t)nw;
+        m.B = (uint32_t)B;
+        m.seg = (uint32_t)sg;
+        m.nseg = (uint32_t)(B / sg);
+        m.bkt_base = (uint32_t)bkt;
+        m.pt_base = (uint32_t)pt;
+        m.rank_base = (uint32_t)rank;
+        m.seg_base = (uint32_t)seg;
+        m.win_base = (uint32_t)win;
+        pt += (uint64_t)s.n;
+        rank += (uint64_t)nw * s.n;
+        bkt += (uint64_t)nw * B;
+        seg += (uint64_t)nw * (B / sg);
+        win += (uint64_t)nw;
+        if (rank >= 0xFFFFFFFFull || bkt >= 0x7FFFFFFFull) return false;
+    }
+    p.n_total = (uint32_t)pt;
+    p.ent_total = (uint32_t)rank;
+    p.bkt_total = (uint32_t)bkt;
+    p.seg_total = (uint32_t)seg;
+    p.win_total = (uint32_t)win;
+    p.chunk = 32;
+    p.lanes = (int)((rank + p.chunk - 1) / p.chunk);
+    size_t cub = 0;
+    if (hipcub::DeviceScan::ExclusiveSum(nullptr, cub, (uint32_t*)nullptr, (uint32_t*)nullptr, (int)(bkt + 1)) !=
+        hipSuccess) {
+        // No device to ask (plan queries on a host without a GPU): a bound on
+        // the decoupled-lookback scan state.
+        cub = (bkt / 128 + 1024) * 16 + 65536;
+    }
+    p.cub_bytes = cub;
+    const size_t xb = xyzz_bytes(curve);
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        const size_t at = o;
+        o += al_size(bytes);
+        return at;
+    };
+    p.o_desc = take((size_t)count * sizeof(MsmDesc));
+    p.o_cnt = take((bkt + 1) * 4);
+    p.o_off = take((bkt + 1) * 4);
+    p.o_rank = take(rank * 4 + 4);
+    p.o_sorted = take(rank * 8 + 8);
+    p.o_buckets = take(bkt * xb);
+    p.o_head = take((size_t)p.lanes * xb);
+    p.o_tail = take((size_t)p.lanes * xb);
+    p.o_segs = take(seg * xb);
+    p.o_wsum = take(win * xb);
+    p.o_out = take((size_t)count * aff_bytes(curve));
+    p.o_err = take(4);
+    p.o_cub = take(cub);
+    p.total = o;
+    return true;
+}
+
+}  // namespace gm

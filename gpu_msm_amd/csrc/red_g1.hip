@@ -6,16 +6,20 @@ namespace gm {
 
 template <class O>
 hipError_t launch_reduce(hipStream_t s, const Plan& p, uint8_t* ws) {
-    k_bucket_reduce<O><<<blocks_for((long)p.nw * p.nseg, 64), 64, 0, s>>>(
-        reinterpret_cast<const Xyzz<O>*>(ws + p.o_buckets), p.nw, p.B, p.seg, reinterpret_cast<Xyzz<O>*>(ws + p.o_segs));
+    if (p.seg_total == 0) return hipSuccess;
+    k_bucket_reduce<O><<<blocks_for((long)p.seg_total, 64), 64, 0, s>>>(
+        reinterpret_cast<const MsmDesc*>(ws + p.o_desc), p.count, p.seg_total,
+        reinterpret_cast<const Xyzz<O>*>(ws + p.o_buckets), reinterpret_cast<Xyzz<O>*>(ws + p.o_segs));
     return hipGetLastError();
 }
 
 template <class O>
 hipError_t launch_window_sum(hipStream_t s, const Plan& p, uint8_t* ws) {
     constexpr int WT = sizeof(Xyzz<O>) > 256 ? 128 : 256;
-    k_window_sum<O, WT><<<p.nw, WT, 0, s>>>(reinterpret_cast<const Xyzz<O>*>(ws + p.o_segs), p.nseg,
-                                            reinterpret_cast<Xyzz<O>*>(ws + p.o_wsum));
+    if (p.win_total == 0) return hipSuccess;
+    k_window_sum<O, WT><<<p.win_total, WT, 0, s>>>(reinterpret_cast<const MsmDesc*>(ws + p.o_desc), p.count,
+                                                   reinterpret_cast<const Xyzz<O>*>(ws + p.o_segs),
+                                                   reinterpret_cast<Xyzz<O>*>(ws + p.o_wsum));
     return hipGetLastError();
 }
 

@@ -7,29 +7,40 @@
 
 namespace gm {
 
+hipError_t launch_fill_u32(hipStream_t s, uint32_t* p, size_t n, uint32_t v) {
+    if (n == 0) return hipSuccess;
+    k_fill_u32<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(p, n, v);
+    return hipGetLastError();
+}
+
 hipError_t launch_sort(hipStream_t s, const Plan& p, const uint64_t* d_scalars, uint8_t* ws, Prof* prof) {
+    const MsmDesc* desc = reinterpret_cast<const MsmDesc*>(ws + p.o_desc);
     uint32_t* cnt = reinterpret_cast<uint32_t*>(ws + p.o_cnt);
     uint32_t* off = reinterpret_cast<uint32_t*>(ws + p.o_off);
     uint32_t* rank = reinterpret_cast<uint32_t*>(ws + p.o_rank);
     uint64_t* sorted = reinterpret_cast<uint64_t*>(ws + p.o_sorted);
     uint32_t* err = reinterpret_cast<uint32_t*>(ws + p.o_err);
-    const size_t nbk = (size_t)p.nw * p.B;
+    const size_t nbk = p.bkt_total;
     auto mark = [&](int st) -> hipError_t {
         return (prof && prof->on) ? hipEventRecord(prof->ev[st], s) : hipSuccess;
     };
-    // counts (+1 sentinel so the scan also yields the total) and error flag
-    GM_TRY(hipMemsetAsync(cnt, 0, (nbk + 1) * sizeof(uint32_t), s));
-    GM_TRY(hipMemsetAsync(err, 0, sizeof(uint32_t), s));
-    if (p.n > 0) {
-        k_digits_hist<<<blocks_for(p.n, 256), 256, 0, s>>>(d_scalars, p.n, p.L, p.c, p.nw, cnt, rank, err);
+    // Counts (+1 sentinel so the scan also yields the total) and the error
+    // flag are cleared by a fill kernel, not hipMemsetAsync, so the sequence
+    // can be captured in a hipGraph and replayed.
+    GM_TRY(launch_fill_u32(s, cnt, nbk + 1, 0));
+    GM_TRY(launch_fill_u32(s, err, 1, 0));
+    if (p.n_total > 0) {
+        k_digits_hist<<<blocks_for(p.n_total, 256), 256, 0, s>>>(desc, p.count, p.n_total, d_scalars, p.L, cnt, rank,
+                                                                 err);
         GM_TRY(hipGetLastError());
     }
     GM_TRY(mark(ST_HIST));
     size_t cub_bytes = p.cub_bytes;
     GM_TRY(hipcub::DeviceScan::ExclusiveSum(ws + p.o_cub, cub_bytes, cnt, off, (int)(nbk + 1), s));
     GM_TRY(mark(ST_SCAN));
-    if (p.n > 0) {
-        k_scatter<<<blocks_for(p.n, 256), 256, 0, s>>>(d_scalars, p.n, p.L, p.c, p.nw, off, rank, sorted);
+    if (p.n_total > 0) {
+        k_scatter<<<blocks_for(p.n_total, 256), 256, 0, s>>>(desc, p.count, p.n_total, d_scalars, p.L, off, rank,
+                                                             sorted);
         GM_TRY(hipGetLastError());
     }
     GM_TRY(mark(ST_SCATTER));

@@ -1,7 +1,11 @@
-// Signed-digit recoding and counting sort of (window, bucket) entries.
+// Signed-digit recoding and counting sort of (window, bucket) entries for a
+// batch of MSMs (one descriptor per MSM).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdint.h>
+
+#include "desc_find.hpp"
 
 namespace gm {
 
@@ -62,42 +66,52 @@ struct DigitIter {
     }
 };
 
-__global__ void __launch_bounds__(256) k_digits_hist(const uint64_t* __restrict__ scalars, int n, int L, int c, int nw,
-                              uint32_t* __restrict__ cnt, uint32_t* __restrict__ rank, uint32_t* __restrict__ err) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__global__ void __launch_bounds__(256) k_digits_hist(const MsmDesc* __restrict__ desc, int count, uint32_t n_total,
+                                                     const uint64_t* __restrict__ scalars, int L,
+                                                     uint32_t* __restrict__ cnt, uint32_t* __restrict__ rank,
+                                                     uint32_t* __restrict__ err) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n_total) return;
+    const MsmDesc& m = desc[GM_FIND(desc, count, pt_base, p)];
+    const uint32_t i = p - m.pt_base;
     DigitIter it;
-    it.load(scalars + (size_t)i * L, L, c);
-    const int B = 1 << (c - 1);
-    for (int w = 0; w < nw; ++w) {
+    it.load(scalars + (size_t)(m.scal_off + i) * L, L, (int)m.c);
+    for (uint32_t w = 0; w < m.nw; ++w) {
         const int d = it.next();
         uint32_t r = 0xFFFFFFFFu;
         if (d != 0) {
-            const int b = (d < 0 ? -d : d) - 1;
-            r = atomicAdd(&cnt[(size_t)w * B + b], 1u);
+            const uint32_t b = (uint32_t)(d < 0 ? -d : d) - 1;
+            r = atomicAdd(&cnt[m.bkt_base + w * m.B + b], 1u);
         }
-        rank[(size_t)w * n + i] = r;
+        rank[(size_t)m.rank_base + (size_t)w * m.n + i] = r;
     }
     if (!it.fits()) atomicOr(err, 1u);
 }
 
-__global__ void __launch_bounds__(256) k_scatter(const uint64_t* __restrict__ scalars, int n, int L, int c, int nw,
-                          const uint32_t* __restrict__ off, const uint32_t* __restrict__ rank,
-                          uint64_t* __restrict__ sorted) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__global__ void __launch_bounds__(256) k_scatter(const MsmDesc* __restrict__ desc, int count, uint32_t n_total,
+                                                 const uint64_t* __restrict__ scalars, int L,
+                                                 const uint32_t* __restrict__ off, const uint32_t* __restrict__ rank,
+                                                 uint64_t* __restrict__ sorted) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n_total) return;
+    const MsmDesc& m = desc[GM_FIND(desc, count, pt_base, p)];
+    const uint32_t i = p - m.pt_base;
     DigitIter it;
-    it.load(scalars + (size_t)i * L, L, c);
-    const int B = 1 << (c - 1);
-    for (int w = 0; w < nw; ++w) {
+    it.load(scalars + (size_t)(m.scal_off + i) * L, L, (int)m.c);
+    for (uint32_t w = 0; w < m.nw; ++w) {
         const int d = it.next();
         if (d != 0) {
-            const int b = (d < 0 ? -d : d) - 1;
-            const uint32_t pos = off[(size_t)w * B + b] + rank[(size_t)w * n + i];
-            // (global bucket << 32) | point index | sign bit
-            sorted[pos] = ((uint64_t)((uint32_t)w * B + b) << 32) | (uint32_t)i | (d < 0 ? 0x80000000u : 0u);
+            const uint32_t g = m.bkt_base + w * m.B + (uint32_t)(d < 0 ? -d : d) - 1;
+            const uint32_t pos = off[g] + rank[(size_t)m.rank_base + (size_t)w * m.n + i];
+            // (global bucket << 32) | base index in the pool | sign bit
+            sorted[pos] = ((uint64_t)g << 32) | (m.base_off + i) | (d < 0 ? 0x80000000u : 0u);
         }
     }
+}
+
+__global__ void __launch_bounds__(256) k_fill_u32(uint32_t* __restrict__ p, size_t n, uint32_t v) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 }  // namespace gm

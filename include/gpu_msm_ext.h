@@ -66,6 +66,22 @@ void gm_scalars_free(gm_scalars* s);
 int gm_msm_resident(void* result_affine, const gm_bases* b, int base_offset, const gm_scalars* s, int scalar_offset,
                     int n, int scalar_bits, int window, int device_combine);
 
+/* One MSM of a batch over resident data. window/seg <= 0: defaults. */
+typedef struct gm_msm_spec {
+    int n;
+    int base_offset;
+    int scalar_offset;
+    int window; /* window bits c */
+    int seg;    /* buckets per bucket-reduction lane (power of two) */
+} gm_msm_spec;
+
+/* Many MSMs of uneven sizes in one pipeline run: every stage is one launch
+ * over all MSMs (one sort, one balanced accumulation over all entries, one
+ * reduction), then one window combine per MSM. results_affine: `count` ABI
+ * affine records. */
+int gm_msm_batch(void* results_affine, const gm_bases* b, const gm_scalars* s, const gm_msm_spec* specs, int count,
+                 int scalar_bits, int device_combine);
+
 /* -------------------------------------------------------------- profiling */
 
 /* Record HIP events between pipeline stages of subsequent MSMs on this GPU. */

@@ -218,3 +218,54 @@ def test_full_size_closed_form(golden, curve, lg):
     off = n // 2 + 3
     sub = gm.msm_resident(b, s, n=1024, base_offset=off, scalar_offset=off, scalar_bits=299)
     assert same(sub, o.msm(curve, o.bases_from_k(curve, ks[off:off + 1024]), ss[off:off + 1024]))
+
+
+# ------------------------------------------------------------------- batched
+
+def _pool(curve, n, seed):
+    ks = o.gen_u64(seed, n)
+    cv = gm.CURVE_G1 if curve == "g1" else gm.CURVE_G2
+    return ks, gm.Bases.from_k(cv, ks)
+
+
+@pytest.mark.parametrize("curve", ["g1", "g2"])
+def test_batch_matches_alone_and_oracle(curve):
+    """Each MSM of a mixed-size batch equals the same MSM run alone, and the
+    small ones equal the oracle."""
+    pool_n = 1 << 14
+    ks, b = _pool(curve, pool_n, 90 if curve == "g1" else 91)
+    rng = np.random.default_rng(5)
+    sizes = [1, 2, 7, 64, 300, 1000, 4096, 1 << 13, 17, 513]
+    total = sum(sizes)
+    ss = o.gen_scalars_modr(92, total)
+    ss[3] = 0
+    s = gm.Scalars.upload(ss)
+    specs, so = [], 0
+    for k, n in enumerate(sizes):
+        bo = int(rng.integers(0, pool_n - n + 1))
+        window = [0, 5, 0, 9, 0, 12, 0, 0, 3, 0][k]
+        specs.append((n, bo, so, window))
+        so += n
+    got = gm.msm_batch(b, s, specs, scalar_bits=299)
+    got_dev = gm.msm_batch(b, s, specs, scalar_bits=299, device_combine=True)
+    assert same(got, got_dev)
+    for k, (n, bo, so_, w) in enumerate(specs):
+        alone = gm.msm_resident(b, s, n=n, base_offset=bo, scalar_offset=so_, scalar_bits=299, window=w)
+        assert same(got[k:k + 1], alone), k
+        if n <= 1000:
+            want = o.msm(curve, o.bases_from_k(curve, ks[bo:bo + n]), ss[so_:so_ + n])
+            assert same(got[k:k + 1], want), k
+
+
+def test_batch_u64_scalars_and_reorder():
+    """Permuting the MSMs of a batch does not change any result."""
+    pool_n = 4096
+    _, b = _pool("g1", pool_n, 93)
+    sc = o.gen_u64(94, 6000).reshape(-1, 1)
+    s = gm.Scalars.upload(sc)
+    specs = [(100, 0, 0), (2000, 50, 100), (5, 4000, 2100), (3000, 1000, 2105), (895, 3000, 5105)]
+    r1 = gm.msm_batch(b, s, specs)
+    perm = [3, 0, 4, 2, 1]
+    r2 = gm.msm_batch(b, s, [specs[k] for k in perm])
+    for a, k in enumerate(perm):
+        assert same(r2[a:a + 1], r1[k:k + 1])
