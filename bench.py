@@ -20,6 +20,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 import gpu_msm_amd as gm  # noqa: E402
+from gpu_msm_amd import workloads as wl  # noqa: E402
 
 R_LIMBS = [0x0428001400040001, 0x7BB9B0E8D8CA3461, 0xD04C98CCC4C050BC, 0x7995B34995830FA4, 0x00000511B70539F2]
 # MI355X: 256 CUs x 64 lanes x 2.4 GHz; v_mad_u64_u32 issues at full rate on
@@ -31,22 +32,7 @@ HBM_PEAK_GBS = 8000.0
 MAD_PER_MIXED_ADD = 11 * 406
 
 
-def gen_scalars(rng: np.random.Generator, n: int) -> np.ndarray:
-    """Uniform in [0, r): 5 x 64-bit draws, top limb masked to 43 bits,
-    rejected when >= r."""
-    out = np.zeros((0, 5), dtype=np.uint64)
-    r = np.array(R_LIMBS, dtype=np.uint64)
-    while len(out) < n:
-        m = int((n - len(out)) * 1.7) + 64
-        v = np.frombuffer(rng.bytes(40 * m), dtype=np.uint64).reshape(m, 5).copy()
-        v[:, 4] &= np.uint64((1 << 43) - 1)
-        lt = np.zeros(m, dtype=bool)
-        eq = np.ones(m, dtype=bool)
-        for j in range(4, -1, -1):
-            lt |= eq & (v[:, j] < r[j])
-            eq &= v[:, j] == r[j]
-        out = np.concatenate([out, v[lt]])
-    return np.ascontiguousarray(out[:n])
+gen_scalars = wl.uniform_scalars
 
 
 def closed_form_scalar(ks: np.ndarray, ss: np.ndarray) -> int:
@@ -93,6 +79,52 @@ def time_msm(bases, scalars, steps: int, warmup: int, profile: bool):
     return res, dt * 1e3 / steps, stages
 
 
+def bench_batch(reps: int) -> dict:
+    """256 G1 MSMs, N log-uniform in [2^8, 2^18], bases slices of one resident
+    pool of 2^18 points: plain (one pipeline run + host combine per MSM, in
+    sequence) against one batched pipeline run, eager and hipGraph replay.
+    Every batched result must equal the same MSM run alone."""
+    pool_k, sizes, boff, soff, sc = wl.batch_workload()
+    pool = gm.Bases.from_k(gm.CURVE_G1, pool_k)
+    s = gm.Scalars.upload(sc)
+    specs = wl.specs_from(sizes, boff, soff)
+    # plain
+    plain = np.zeros(len(specs), dtype=gm.G1_DTYPE)
+    for k, sp in enumerate(specs[:8]):  # warm-up
+        gm.msm_resident(pool, s, n=sp[0], base_offset=sp[1], scalar_offset=sp[2], scalar_bits=299)
+    t0 = time.perf_counter()
+    for k, sp in enumerate(specs):
+        plain[k] = gm.msm_resident(pool, s, n=sp[0], base_offset=sp[1], scalar_offset=sp[2], scalar_bits=299)[0]
+    t_plain = time.perf_counter() - t0
+    # batched, eager launches
+    gm.msm_batch(pool, s, specs, scalar_bits=299)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        eager = gm.msm_batch(pool, s, specs, scalar_bits=299)
+    t_eager = (time.perf_counter() - t0) / reps
+    # batched, one hipGraph replay per run
+    bt = gm.Batch(pool, s, specs, scalar_bits=299, use_graph=True)
+    bt.run()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        graph = bt.run()
+    t_graph = (time.perf_counter() - t0) / reps
+    ok = plain.tobytes() == eager.tobytes() == graph.tobytes()
+    n = len(specs)
+    out = {
+        "msms": n, "points": int(sizes.sum()),
+        "plain_msms_per_s": round(n / t_plain, 1),
+        "batched_msms_per_s": round(n / t_eager, 1),
+        "graph_msms_per_s": round(n / t_graph, 1),
+        "speedup_vs_plain": round(t_plain / min(t_eager, t_graph), 2),
+        "plain_ms": round(t_plain * 1e3, 2), "batched_ms": round(t_eager * 1e3, 2), "graph_ms": round(t_graph * 1e3, 2),
+        "workspace_GB": round(bt.workspace_bytes / 1e9, 2),
+        "results_equal_alone": ok,
+    }
+    bt.free()
+    return out
+
+
 def cpu_baseline(bases_dev, scalars_np: np.ndarray, curve: str, n_full: int):
     """Oracle host Pippenger on a bounded sample (N = 2^16, window 16) on all
     host cores, scaled to N = 2^20 by the window-16 cost model
@@ -125,6 +157,8 @@ def main() -> None:
     ap.add_argument("--log2n", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-g2", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--no-self-check", action="store_true",
                     help="skip the closed-form check (its n=1 MSM would mix into profiler averages)")
     args = ap.parse_args()
@@ -166,6 +200,11 @@ def main() -> None:
                                                  for k in gm.STAGES}}
         b2.free()
 
+    batched = None
+    if not args.no_batch:
+        print("bench: batched workload (256 MSMs, N log-uniform in [2^8, 2^18])", file=sys.stderr, flush=True)
+        batched = bench_batch(args.batch_reps)
+
     cpu = None
     if not args.no_cpu_baseline:
         print("bench: cpu baseline (oracle, bounded sample)", file=sys.stderr, flush=True)
@@ -203,6 +242,7 @@ def main() -> None:
             "launch_ms": round(acc_ms, 3),
             "mixed_adds_per_launch": info["entries"],
         },
+        "batched": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -53,6 +53,8 @@ EXPORTS = [
     "init_device_generators", "g1_msm_multi_wrapper", "g2_msm_multi_wrapper", "gm_plan_query",
     "gm_bases_upload", "gm_bases_from_k", "gm_bases_download", "gm_bases_count", "gm_bases_free",
     "gm_scalars_upload", "gm_scalars_free", "gm_msm_resident", "gm_msm_batch",
+    "gm_batch_create", "gm_batch_launch", "gm_batch_wait", "gm_batch_run", "gm_batch_count",
+    "gm_batch_workspace_bytes", "gm_batch_free",
     "gm_set_profiling", "gm_profile_last", "gm_last_msm_info",
 ]
 
@@ -102,6 +104,13 @@ def lib() -> ctypes.CDLL:
         "gm_scalars_free": ([vp], None),
         "gm_msm_resident": ([vp, vp, i, vp, i, i, i, i, i], i),
         "gm_msm_batch": ([vp, vp, vp, vp, i, i, i], i),
+        "gm_batch_create": ([ctypes.POINTER(vp), vp, vp, vp, i, i, i], i),
+        "gm_batch_launch": ([vp], i),
+        "gm_batch_wait": ([vp, vp], i),
+        "gm_batch_run": ([vp, vp], i),
+        "gm_batch_count": ([vp], i),
+        "gm_batch_workspace_bytes": ([vp], sz),
+        "gm_batch_free": ([vp], None),
         "gm_set_profiling": ([u32, i], i),
         "gm_profile_last": ([u32, ctypes.POINTER(ctypes.c_float), i], i),
         "gm_last_msm_info": ([u32, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(ctypes.c_longlong)], i),
@@ -349,17 +358,61 @@ SPEC_DTYPE = np.dtype([("n", "<i4"), ("base_offset", "<i4"), ("scalar_offset", "
 def msm_batch(bases: Bases, scalars: Scalars, specs, scalar_bits: int = 0, device_combine: bool = False) -> np.ndarray:
     """Many MSMs in one pipeline run. specs: array of SPEC_DTYPE or sequence of
     (n, base_offset, scalar_offset[, window[, seg]]). Returns affine ABI records."""
-    sp = np.zeros(len(specs), dtype=SPEC_DTYPE)
-    if isinstance(specs, np.ndarray) and specs.dtype == SPEC_DTYPE:
-        sp[:] = specs
-    else:
-        for k, s in enumerate(specs):
-            s = tuple(s) + (0,) * (5 - len(s))
-            sp[k] = s
+    sp = _spec_array(specs)
     out = np.zeros(len(sp), dtype=G1_DTYPE if bases.curve == CURVE_G1 else G2_DTYPE)
     _check(lib().gm_msm_batch(_ptr(out), bases.h, scalars.h, _ptr(sp), len(sp), scalar_bits, int(device_combine)),
            "batched MSM")
     return out
+
+
+def _spec_array(specs) -> np.ndarray:
+    if isinstance(specs, np.ndarray) and specs.dtype == SPEC_DTYPE:
+        return np.ascontiguousarray(specs)
+    sp = np.zeros(len(specs), dtype=SPEC_DTYPE)
+    for k, s in enumerate(specs):
+        sp[k] = tuple(s) + (0,) * (5 - len(s))
+    return sp
+
+
+class Batch:
+    """A fixed batch of MSMs over resident data, optionally captured in a
+    hipGraph once and replayed on every run."""
+
+    def __init__(self, bases: Bases, scalars: Scalars, specs, scalar_bits: int = 0, use_graph: bool = True):
+        self.specs = _spec_array(specs)
+        self.bases, self.scalars = bases, scalars  # keep the handles alive
+        self.curve = bases.curve
+        h = ctypes.c_void_p()
+        _check(lib().gm_batch_create(ctypes.byref(h), bases.h, scalars.h, _ptr(self.specs), len(self.specs),
+                                     scalar_bits, int(use_graph)), "batch create")
+        self.h = h.value
+        self.out = np.zeros(len(self.specs), dtype=G1_DTYPE if self.curve == CURVE_G1 else G2_DTYPE)
+
+    def run(self) -> np.ndarray:
+        _check(lib().gm_batch_run(self.h, _ptr(self.out)), "batch run")
+        return self.out.copy()
+
+    def launch(self) -> None:
+        _check(lib().gm_batch_launch(self.h), "batch launch")
+
+    def wait(self) -> np.ndarray:
+        _check(lib().gm_batch_wait(self.h, _ptr(self.out)), "batch wait")
+        return self.out.copy()
+
+    @property
+    def workspace_bytes(self) -> int:
+        return int(lib().gm_batch_workspace_bytes(self.h))
+
+    def free(self) -> None:
+        if getattr(self, "h", None):
+            lib().gm_batch_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 def plan_query(curve: int, n: int, limbs: int, bits: int = 0, window: int = 0):

@@ -18,6 +18,12 @@
 #include "engine.hpp"
 #include "host_field.hpp"
 
+#define GM_TRY_HOST(x)                   \
+    do {                                 \
+        hipError_t e_ = (x);             \
+        if (e_ != hipSuccess) return e_; \
+    } while (0)
+
 using gm::Aff;
 using gm::Fp2Ops;
 using gm::FpOps;
@@ -440,6 +446,78 @@ int resident_batch(void* results_affine, const gm_bases* b, const gm_scalars* s,
     return 0;
 }
 
+// ---------------------------------------------------------- batch objects
+
+}  // namespace
+
+// A fixed batch of MSMs over resident data with its own stream, workspace and
+// (optionally) a captured hipGraph of the whole pipeline, replayed per run.
+struct gm_batch {
+    int curve = 0, dev = 0, count = 0;
+    std::vector<gm::MsmDesc> desc;
+    Plan p{};
+    const gm_bases* b = nullptr;
+    const gm_scalars* s = nullptr;
+    uint8_t* ws = nullptr;
+    uint8_t* pinned = nullptr;
+    size_t res_bytes = 0;
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool use_graph = false;
+    bool in_flight = false;
+};
+
+namespace {
+
+template <class O>
+hipError_t batch_enqueue(gm_batch* bt) {
+    GM_TRY_HOST(gm::Engine<O>::run(bt->stream, bt->p, reinterpret_cast<const Aff<O>*>(bt->b->d), bt->s->d, bt->ws,
+                                   nullptr));
+    GM_TRY_HOST(gm::Engine<O>::device_combine(bt->stream, bt->p, bt->ws));
+    GM_TRY_HOST(hipMemcpyAsync(bt->pinned, bt->ws + bt->p.o_out, bt->res_bytes, hipMemcpyDeviceToHost, bt->stream));
+    GM_TRY_HOST(hipMemcpyAsync(bt->pinned + gm::al_size(bt->res_bytes), bt->ws + bt->p.o_err, 4,
+                               hipMemcpyDeviceToHost, bt->stream));
+    return hipSuccess;
+}
+
+hipError_t batch_enqueue_any(gm_batch* bt) {
+    return bt->curve == GM_CURVE_G1 ? batch_enqueue<FpOps>(bt) : batch_enqueue<Fp2Ops>(bt);
+}
+
+void batch_release(gm_batch* bt) {
+    if (!bt) return;
+    hipSetDevice(bt->dev);
+    if (bt->stream) hipStreamSynchronize(bt->stream);
+    if (bt->exec) hipGraphExecDestroy(bt->exec);
+    if (bt->graph) hipGraphDestroy(bt->graph);
+    if (bt->stream) hipStreamDestroy(bt->stream);
+    if (bt->ws) hipFree(bt->ws);
+    if (bt->pinned) hipHostFree(bt->pinned);
+    delete bt;
+}
+
+template <class O>
+int batch_results(gm_batch* bt, void* results_affine) {
+    using HF = typename HostOf<O>::HF;
+    uint32_t err;
+    memcpy(&err, bt->pinned + gm::al_size(bt->res_bytes), 4);
+    if (err) return -5;
+    constexpr int W = words_of<HF>();
+    const size_t rec = HostOf<O>::CURVE == gm::CURVE_G1 ? 120 : 232;
+    for (int j = 0; j < bt->count; ++j) {
+        const uint32_t* a = reinterpret_cast<const uint32_t*>(bt->pinned + (size_t)j * sizeof(Aff<O>));
+        HF x{}, y{};
+        const bool finite = a[15] != gm::INF_MARK;
+        if (finite) {
+            dev_elem_to_host(x, a);
+            dev_elem_to_host(y, a + W);
+        }
+        write_affine_abi<HF>(reinterpret_cast<uint8_t*>(results_affine) + (size_t)j * rec, finite, x, y);
+    }
+    return 0;
+}
+
 }  // namespace
 
 // ================================================================= C ABI
@@ -682,6 +760,93 @@ int gm_msm_batch(void* results_affine, const gm_bases* b, const gm_scalars* s, c
         return resident_batch<FpOps>(results_affine, b, s, specs, count, scalar_bits, device_combine);
     return resident_batch<Fp2Ops>(results_affine, b, s, specs, count, scalar_bits, device_combine);
 }
+
+int gm_batch_create(gm_batch** out, const gm_bases* b, const gm_scalars* s, const gm_msm_spec* specs, int count,
+                    int scalar_bits, int use_graph) {
+    if (!out || !b || !s || !specs || count <= 0 || b->dev != s->dev) return -1;
+    *out = nullptr;
+    std::vector<gm::MsmSpec> sp((size_t)count);
+    for (int j = 0; j < count; ++j) {
+        const gm_msm_spec& q = specs[j];
+        if (q.n <= 0 || q.base_offset < 0 || q.scalar_offset < 0) return -1;
+        if ((long)q.base_offset + q.n > b->n || (long)q.scalar_offset + q.n > s->n) return -1;
+        sp[j] = {q.n, q.base_offset, q.scalar_offset, q.window, q.seg};
+    }
+    if (hipSetDevice(b->dev) != hipSuccess) return -2;
+    gm_batch* bt = new gm_batch();
+    bt->curve = b->curve;
+    bt->dev = b->dev;
+    bt->count = count;
+    bt->b = b;
+    bt->s = s;
+    bt->desc.resize((size_t)count);
+    if (!gm::make_plan(bt->p, bt->desc.data(), b->curve, s->limbs, scalar_bits, sp.data(), count)) {
+        delete bt;
+        return -1;
+    }
+    bt->res_bytes = (size_t)count * gm::aff_bytes(b->curve);
+    if (hipStreamCreateWithFlags(&bt->stream, hipStreamNonBlocking) != hipSuccess) {
+        batch_release(bt);
+        return -2;
+    }
+    if (hipMalloc(&bt->ws, bt->p.total) != hipSuccess ||
+        hipHostMalloc((void**)&bt->pinned, gm::al_size(bt->res_bytes) + 256, hipHostMallocDefault) != hipSuccess) {
+        batch_release(bt);
+        return -3;
+    }
+    if (hipMemcpy(bt->ws + bt->p.o_desc, bt->desc.data(), (size_t)count * sizeof(gm::MsmDesc),
+                  hipMemcpyHostToDevice) != hipSuccess) {
+        batch_release(bt);
+        return -4;
+    }
+    if (use_graph) {
+        hipError_t e = hipStreamBeginCapture(bt->stream, hipStreamCaptureModeRelaxed);
+        if (e == hipSuccess) {
+            hipError_t e2 = batch_enqueue_any(bt);
+            e = hipStreamEndCapture(bt->stream, &bt->graph);
+            if (e2 != hipSuccess) e = e2;
+        }
+        if (e == hipSuccess) e = hipGraphInstantiate(&bt->exec, bt->graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            batch_release(bt);
+            return -4;
+        }
+        bt->use_graph = true;
+    }
+    *out = bt;
+    return 0;
+}
+
+int gm_batch_launch(gm_batch* bt) {
+    if (!bt || bt->in_flight) return -1;
+    if (hipSetDevice(bt->dev) != hipSuccess) return -2;
+    hipError_t e = bt->use_graph ? hipGraphLaunch(bt->exec, bt->stream) : batch_enqueue_any(bt);
+    if (e != hipSuccess) return -4;
+    bt->in_flight = true;
+    return 0;
+}
+
+int gm_batch_wait(gm_batch* bt, void* results_affine) {
+    if (!bt || !bt->in_flight || !results_affine) return -1;
+    if (hipSetDevice(bt->dev) != hipSuccess) return -2;
+    hipError_t e = hipStreamSynchronize(bt->stream);
+    bt->in_flight = false;
+    if (e != hipSuccess) return -4;
+    return bt->curve == GM_CURVE_G1 ? batch_results<FpOps>(bt, results_affine)
+                                    : batch_results<Fp2Ops>(bt, results_affine);
+}
+
+int gm_batch_run(gm_batch* bt, void* results_affine) {
+    int rc = gm_batch_launch(bt);
+    if (rc) return rc;
+    return gm_batch_wait(bt, results_affine);
+}
+
+int gm_batch_count(const gm_batch* bt) { return bt ? bt->count : 0; }
+
+size_t gm_batch_workspace_bytes(const gm_batch* bt) { return bt ? bt->p.total : 0; }
+
+void gm_batch_free(gm_batch* bt) { batch_release(bt); }
 
 int gm_set_profiling(uint32_t gpu_index, int on) {
     DevCtx* c = get_ctx(gpu_index);

@@ -82,6 +82,22 @@ typedef struct gm_msm_spec {
 int gm_msm_batch(void* results_affine, const gm_bases* b, const gm_scalars* s, const gm_msm_spec* specs, int count,
                  int scalar_bits, int device_combine);
 
+/* A fixed batch bound to resident bases and scalars, with its own stream and
+ * device workspace. use_graph != 0 captures the whole pipeline (sort,
+ * accumulation, reduction, device window combine, result copy) in a hipGraph
+ * at creation; every run replays it. The scalars' contents may change between
+ * runs; the handles must outlive the batch. One batch must not be launched
+ * again before gm_batch_wait returns; different batches may overlap. */
+typedef struct gm_batch gm_batch;
+int gm_batch_create(gm_batch** out, const gm_bases* b, const gm_scalars* s, const gm_msm_spec* specs, int count,
+                    int scalar_bits, int use_graph);
+int gm_batch_launch(gm_batch* bt);                       /* asynchronous */
+int gm_batch_wait(gm_batch* bt, void* results_affine);   /* count ABI affine records */
+int gm_batch_run(gm_batch* bt, void* results_affine);    /* launch + wait */
+int gm_batch_count(const gm_batch* bt);
+size_t gm_batch_workspace_bytes(const gm_batch* bt);
+void gm_batch_free(gm_batch* bt);
+
 /* -------------------------------------------------------------- profiling */
 
 /* Record HIP events between pipeline stages of subsequent MSMs on this GPU. */

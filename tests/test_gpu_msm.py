@@ -269,3 +269,31 @@ def test_batch_u64_scalars_and_reorder():
     r2 = gm.msm_batch(b, s, [specs[k] for k in perm])
     for a, k in enumerate(perm):
         assert same(r2[a:a + 1], r1[k:k + 1])
+
+
+def test_batch_graph_replays_match_eager():
+    """A captured hipGraph of the batch pipeline gives the eager result on
+    every replay, with eager work and synchronisation between replays."""
+    pool_n = 1 << 13
+    _, b = _pool("g1", pool_n, 95)
+    sizes = [3000, 17, 900, 4096, 1]
+    ss = o.gen_scalars_modr(96, sum(sizes))
+    s = gm.Scalars.upload(ss)
+    specs, so = [], 0
+    for k, n in enumerate(sizes):
+        specs.append((n, (k * 997) % (pool_n - n + 1), so))
+        so += n
+    eager = gm.msm_batch(b, s, specs, scalar_bits=299)
+    bt = gm.Batch(b, s, specs, scalar_bits=299, use_graph=True)
+    nog = gm.Batch(b, s, specs, scalar_bits=299, use_graph=False)
+    for rep in range(4):
+        assert same(bt.run(), eager), rep
+        assert same(nog.run(), eager), rep
+        gm.msm_resident(b, s, n=100, scalar_bits=299)  # eager work between replays
+    # launch/wait split and a second batch overlapping on the device
+    bt.launch()
+    nog.launch()
+    assert same(bt.wait(), eager)
+    assert same(nog.wait(), eager)
+    bt.free()
+    nog.free()
