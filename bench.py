@@ -111,6 +111,8 @@ def bench_batch(reps: int) -> dict:
     t_graph = (time.perf_counter() - t0) / reps
     ok = plain.tobytes() == eager.tobytes() == graph.tobytes()
     n = len(specs)
+    # planner A/B on the same batch: analytic heuristic vs autotuned table vs MLP
+    ab = planner_ab(pool, s, sizes, boff, soff, graph, reps)
     out = {
         "msms": n, "points": int(sizes.sum()),
         "plain_msms_per_s": round(n / t_plain, 1),
@@ -120,8 +122,39 @@ def bench_batch(reps: int) -> dict:
         "plain_ms": round(t_plain * 1e3, 2), "batched_ms": round(t_eager * 1e3, 2), "graph_ms": round(t_graph * 1e3, 2),
         "workspace_GB": round(bt.workspace_bytes / 1e9, 2),
         "results_equal_alone": ok,
+        "planners": ab,
     }
     bt.free()
+    return out
+
+
+def planner_ab(pool, s, sizes, boff, soff, reference: np.ndarray, reps: int) -> dict:
+    """Times the batch under each planner (hipGraph replay, host combine);
+    every plan must give the same results."""
+    from gpu_msm_amd import planner as pl
+
+    planners = [pl.HeuristicPlanner()]
+    tdir = pl.TUNING_DIR
+    if os.path.exists(os.path.join(tdir, "sweep.json")):
+        planners.append(pl.TablePlanner.load(os.path.join(tdir, "sweep.json")))
+    if os.path.exists(os.path.join(tdir, "mlp.json")):
+        planners.append(pl.MlpPlanner.load(os.path.join(tdir, "mlp.json")))
+    out = {}
+    for p in planners:
+        w, sg = p.plan(gm.CURVE_G1, sizes, len(sizes))
+        bt = gm.Batch(pool, s, wl.specs_from(sizes, boff, soff, w, sg), scalar_bits=299, use_graph=True)
+        res = bt.run()
+        best = 1e30
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            res = bt.run()
+            best = min(best, time.perf_counter() - t0)
+        bt.free()
+        out[p.name] = {"ms": round(best * 1e3, 2), "msms_per_s": round(len(sizes) / best, 1),
+                       "same_results": res.tobytes() == reference.tobytes()}
+    if "heuristic" in out:
+        for k in out:
+            out[k]["speedup_vs_heuristic"] = round(out["heuristic"]["ms"] / out[k]["ms"], 3)
     return out
 
 

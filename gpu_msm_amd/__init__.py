@@ -30,6 +30,8 @@ CURVE_G1 = 1
 CURVE_G2 = 2
 POINTS_MONTGOMERY = 0x1
 SCALARS_LT_R = 0x2
+BATCH_GRAPH = 0x1
+BATCH_DEVICE_COMBINE = 0x2
 R_ORDER = 0x511B70539F27995B34995830FA4D04C98CCC4C050BC7BB9B0E8D8CA34610428001400040001
 R_BITS = 299
 
@@ -54,7 +56,7 @@ EXPORTS = [
     "gm_bases_upload", "gm_bases_from_k", "gm_bases_download", "gm_bases_count", "gm_bases_free",
     "gm_scalars_upload", "gm_scalars_free", "gm_msm_resident", "gm_msm_batch",
     "gm_batch_create", "gm_batch_launch", "gm_batch_wait", "gm_batch_run", "gm_batch_count",
-    "gm_batch_workspace_bytes", "gm_batch_free",
+    "gm_batch_workspace_bytes", "gm_batch_free", "gm_mlp_forward", "gm_mlp_grad", "gm_mlp_train",
     "gm_set_profiling", "gm_profile_last", "gm_last_msm_info",
 ]
 
@@ -111,6 +113,9 @@ def lib() -> ctypes.CDLL:
         "gm_batch_count": ([vp], i),
         "gm_batch_workspace_bytes": ([vp], sz),
         "gm_batch_free": ([vp], None),
+        "gm_mlp_forward": ([vp, vp, i, vp, u32], i),
+        "gm_mlp_grad": ([vp, vp, vp, i, vp, vp, u32], i),
+        "gm_mlp_train": ([vp, vp, vp, i, i, ctypes.c_float, vp, u32], i),
         "gm_set_profiling": ([u32, i], i),
         "gm_profile_last": ([u32, ctypes.POINTER(ctypes.c_float), i], i),
         "gm_last_msm_info": ([u32, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(ctypes.c_longlong)], i),
@@ -378,13 +383,15 @@ class Batch:
     """A fixed batch of MSMs over resident data, optionally captured in a
     hipGraph once and replayed on every run."""
 
-    def __init__(self, bases: Bases, scalars: Scalars, specs, scalar_bits: int = 0, use_graph: bool = True):
+    def __init__(self, bases: Bases, scalars: Scalars, specs, scalar_bits: int = 0, use_graph: bool = True,
+                 device_combine: bool = False):
         self.specs = _spec_array(specs)
         self.bases, self.scalars = bases, scalars  # keep the handles alive
         self.curve = bases.curve
         h = ctypes.c_void_p()
+        flags = (BATCH_GRAPH if use_graph else 0) | (BATCH_DEVICE_COMBINE if device_combine else 0)
         _check(lib().gm_batch_create(ctypes.byref(h), bases.h, scalars.h, _ptr(self.specs), len(self.specs),
-                                     scalar_bits, int(use_graph)), "batch create")
+                                     scalar_bits, flags), "batch create")
         self.h = h.value
         self.out = np.zeros(len(self.specs), dtype=G1_DTYPE if self.curve == CURVE_G1 else G2_DTYPE)
 

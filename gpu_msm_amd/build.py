@@ -23,18 +23,27 @@ FLAGS = [
 ]
 
 
-def _headers_mtime() -> float:
-    hs = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hpp", ".h"))]
-    inc = os.path.join(os.path.dirname(HERE), "include")
-    hs += [os.path.join(inc, f) for f in os.listdir(inc)]
-    return max(os.path.getmtime(h) for h in hs)
+def _deps(path: str, seen: set | None = None) -> set:
+    """Quoted #include dependencies of a source, recursively."""
+    import re
+
+    seen = set() if seen is None else seen
+    for m in re.finditer(r'^\s*#include\s+"([^"]+)"', open(path).read(), flags=re.M):
+        dep = os.path.normpath(os.path.join(os.path.dirname(path), m.group(1)))
+        if dep not in seen and os.path.exists(dep):
+            seen.add(dep)
+            _deps(dep, seen)
+    return seen
+
+
+def _inputs_mtime(src_path: str) -> float:
+    return max(os.path.getmtime(p) for p in {src_path, *_deps(src_path)})
 
 
 def _compile(src: str, incremental: bool = False) -> str:
     obj = os.path.join(BUILD, src.replace(".hip", ".o"))
     if incremental and os.path.exists(obj):
-        t = os.path.getmtime(obj)
-        if t > os.path.getmtime(os.path.join(CSRC, src)) and t > _headers_mtime():
+        if os.path.getmtime(obj) > _inputs_mtime(os.path.join(CSRC, src)):
             return obj
     cmd = [HIPCC, *FLAGS, "-I", os.path.join(os.path.dirname(HERE), "include"), "-c",
            os.path.join(CSRC, src), "-o", obj]

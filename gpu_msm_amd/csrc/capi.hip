@@ -260,24 +260,7 @@ int run_batch_locked(DevCtx* c, const Aff<O>* d_bases, const uint64_t* d_scalars
             }
         }
     } else {
-        const uint32_t* raw = reinterpret_cast<const uint32_t*>(h + o_res);
-        auto work = [&](int j0, int j1) {
-            for (int j = j0; j < j1; ++j)
-                out[j].finite = host_combine<HF>(out[j].x, out[j].y, raw + (size_t)desc[j].win_base * 4 * W,
-                                                 (int)desc[j].nw, (int)desc[j].c);
-        };
-        const int nt = count < 8 ? 1 : (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
-        if (nt == 1) {
-            work(0, count);
-        } else {
-            std::vector<std::thread> th;
-            const int per = (count + nt - 1) / nt;
-            for (int t = 0; t < nt; ++t) {
-                const int j0 = t * per, j1 = std::min(count, j0 + per);
-                if (j0 < j1) th.emplace_back(work, j0, j1);
-            }
-            for (auto& t : th) t.join();
-        }
+        host_combine_many<HF>(out, reinterpret_cast<const uint32_t*>(h + o_res), desc.data(), count);
     }
     return 0;
 }
@@ -465,6 +448,7 @@ struct gm_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool use_graph = false;
+    bool device_combine = false;
     bool in_flight = false;
 };
 
@@ -474,8 +458,14 @@ template <class O>
 hipError_t batch_enqueue(gm_batch* bt) {
     GM_TRY_HOST(gm::Engine<O>::run(bt->stream, bt->p, reinterpret_cast<const Aff<O>*>(bt->b->d), bt->s->d, bt->ws,
                                    nullptr));
-    GM_TRY_HOST(gm::Engine<O>::device_combine(bt->stream, bt->p, bt->ws));
-    GM_TRY_HOST(hipMemcpyAsync(bt->pinned, bt->ws + bt->p.o_out, bt->res_bytes, hipMemcpyDeviceToHost, bt->stream));
+    if (bt->device_combine) {
+        GM_TRY_HOST(gm::Engine<O>::device_combine(bt->stream, bt->p, bt->ws));
+        GM_TRY_HOST(
+            hipMemcpyAsync(bt->pinned, bt->ws + bt->p.o_out, bt->res_bytes, hipMemcpyDeviceToHost, bt->stream));
+    } else {
+        GM_TRY_HOST(
+            hipMemcpyAsync(bt->pinned, bt->ws + bt->p.o_wsum, bt->res_bytes, hipMemcpyDeviceToHost, bt->stream));
+    }
     GM_TRY_HOST(hipMemcpyAsync(bt->pinned + gm::al_size(bt->res_bytes), bt->ws + bt->p.o_err, 4,
                                hipMemcpyDeviceToHost, bt->stream));
     return hipSuccess;
@@ -497,6 +487,30 @@ void batch_release(gm_batch* bt) {
     delete bt;
 }
 
+// Host window combines of many MSMs, spread over host threads.
+template <class HF>
+void host_combine_many(std::vector<AffResult<HF>>& out, const uint32_t* raw, const gm::MsmDesc* desc, int count) {
+    constexpr int W = words_of<HF>();
+    out.resize((size_t)count);
+    auto work = [&](int j0, int j1) {
+        for (int j = j0; j < j1; ++j)
+            out[j].finite = host_combine<HF>(out[j].x, out[j].y, raw + (size_t)desc[j].win_base * 4 * W,
+                                             (int)desc[j].nw, (int)desc[j].c);
+    };
+    const int nt = count < 8 ? 1 : (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
+    if (nt == 1) {
+        work(0, count);
+        return;
+    }
+    std::vector<std::thread> th;
+    const int per = (count + nt - 1) / nt;
+    for (int t = 0; t < nt; ++t) {
+        const int j0 = t * per, j1 = std::min(count, j0 + per);
+        if (j0 < j1) th.emplace_back(work, j0, j1);
+    }
+    for (auto& t : th) t.join();
+}
+
 template <class O>
 int batch_results(gm_batch* bt, void* results_affine) {
     using HF = typename HostOf<O>::HF;
@@ -505,16 +519,22 @@ int batch_results(gm_batch* bt, void* results_affine) {
     if (err) return -5;
     constexpr int W = words_of<HF>();
     const size_t rec = HostOf<O>::CURVE == gm::CURVE_G1 ? 120 : 232;
-    for (int j = 0; j < bt->count; ++j) {
-        const uint32_t* a = reinterpret_cast<const uint32_t*>(bt->pinned + (size_t)j * sizeof(Aff<O>));
-        HF x{}, y{};
-        const bool finite = a[15] != gm::INF_MARK;
-        if (finite) {
-            dev_elem_to_host(x, a);
-            dev_elem_to_host(y, a + W);
+    std::vector<AffResult<HF>> res((size_t)bt->count);
+    if (bt->device_combine) {
+        for (int j = 0; j < bt->count; ++j) {
+            const uint32_t* a = reinterpret_cast<const uint32_t*>(bt->pinned + (size_t)j * sizeof(Aff<O>));
+            res[j].finite = a[15] != gm::INF_MARK;
+            if (res[j].finite) {
+                dev_elem_to_host(res[j].x, a);
+                dev_elem_to_host(res[j].y, a + W);
+            }
         }
-        write_affine_abi<HF>(reinterpret_cast<uint8_t*>(results_affine) + (size_t)j * rec, finite, x, y);
+    } else {
+        host_combine_many<HF>(res, reinterpret_cast<const uint32_t*>(bt->pinned), bt->desc.data(), bt->count);
     }
+    for (int j = 0; j < bt->count; ++j)
+        write_affine_abi<HF>(reinterpret_cast<uint8_t*>(results_affine) + (size_t)j * rec, res[j].finite, res[j].x,
+                             res[j].y);
     return 0;
 }
 
@@ -762,7 +782,8 @@ int gm_msm_batch(void* results_affine, const gm_bases* b, const gm_scalars* s, c
 }
 
 int gm_batch_create(gm_batch** out, const gm_bases* b, const gm_scalars* s, const gm_msm_spec* specs, int count,
-                    int scalar_bits, int use_graph) {
+                    int scalar_bits, int flags) {
+    const int use_graph = (flags & GM_BATCH_GRAPH) != 0;
     if (!out || !b || !s || !specs || count <= 0 || b->dev != s->dev) return -1;
     *out = nullptr;
     std::vector<gm::MsmSpec> sp((size_t)count);
@@ -784,7 +805,9 @@ int gm_batch_create(gm_batch** out, const gm_bases* b, const gm_scalars* s, cons
         delete bt;
         return -1;
     }
-    bt->res_bytes = (size_t)count * gm::aff_bytes(b->curve);
+    bt->device_combine = (flags & GM_BATCH_DEVICE_COMBINE) != 0;
+    bt->res_bytes = bt->device_combine ? (size_t)count * gm::aff_bytes(b->curve)
+                                       : (size_t)bt->p.win_total * gm::xyzz_bytes(b->curve);
     if (hipStreamCreateWithFlags(&bt->stream, hipStreamNonBlocking) != hipSuccess) {
         batch_release(bt);
         return -2;

@@ -52,7 +52,9 @@ struct MsmDesc {
     uint32_t rank_base;  // first (window, point) rank slot
     uint32_t seg_base;   // first global segment
     uint32_t win_base;   // first global window
-    uint32_t pad[3];
+    uint32_t chunk;      // entries per accumulation lane for this MSM
+    uint32_t lane_base;  // first accumulation lane
+    uint32_t pad;
 };
 static_assert(sizeof(MsmDesc) == 64, "MsmDesc layout");
 
@@ -71,11 +73,11 @@ struct Plan {
     int bits;       // scalar bit bound (values must be < 2^bits)
     int count;      // MSMs in the batch
     uint32_t n_total, ent_total, bkt_total, seg_total, win_total;
-    int chunk;      // entries per accumulation lane
     int lanes;      // accumulation lanes (upper bound: all digits non-zero)
     size_t cub_bytes;
     // workspace offsets (bytes)
-    size_t o_desc, o_cnt, o_off, o_rank, o_sorted, o_buckets, o_head, o_tail, o_segs, o_wsum, o_out, o_err, o_cub,
+    size_t o_desc, o_cnt, o_off, o_rank, o_sorted, o_buckets, o_head, o_tail, o_heavy, o_segs, o_wsum, o_out, o_err,
+        o_cub,
         total;
     MsmDesc* host_desc;  // owned by the caller of make_plan (count entries)
 };

@@ -169,23 +169,27 @@ __global__ void __launch_bounds__(64) k_gen_bases(Aff<O>* __restrict__ out, cons
 // ------------------------------------------------------ bucket accumulation
 
 // Entries are 64-bit (global bucket << 32) | point index | sign << 31, sorted
-// so each bucket's entries are contiguous (offsets in off[]). Every lane takes
-// a fixed chunk of K consecutive entries, whatever the bucket sizes, so lanes
-// stay balanced even when a window has few, very large buckets (the top window
-// of a 299-bit scalar holds only ~11 bits). A run of one bucket that lies
-// wholly inside a chunk is written to its bucket; a run cut by the chunk's
-// start goes to head[lane], one cut only by its end to tail[lane], and
-// k_bucket_fixup sums those partials.
+// so each bucket's entries are contiguous (offsets in off[]) and each MSM's
+// buckets follow each other. Every lane takes a fixed chunk of its MSM's
+// entries (desc.chunk of them), whatever the bucket sizes, so lanes stay
+// balanced even when a window has few, very large buckets (the top window of
+// a 299-bit scalar holds only a few bits). A run of one bucket wholly inside a
+// chunk is written to its bucket; a run cut by the chunk's start goes to
+// head[lane], one cut only by its end to tail[lane], and k_bucket_fixup sums
+// those partials.
 template <class O>
-__global__ void __launch_bounds__(64) k_accumulate(const Aff<O>* __restrict__ bases, const uint64_t* __restrict__ entries,
-                                                   const uint32_t* __restrict__ off, int nbk, int K,
-                                                   Xyzz<O>* __restrict__ buckets, Xyzz<O>* __restrict__ head,
-                                                   Xyzz<O>* __restrict__ tail) {
+__global__ void __launch_bounds__(64) k_accumulate(const MsmDesc* __restrict__ desc, int count, uint32_t lanes,
+                                                   const Aff<O>* __restrict__ bases,
+                                                   const uint64_t* __restrict__ entries,
+                                                   const uint32_t* __restrict__ off, Xyzz<O>* __restrict__ buckets,
+                                                   Xyzz<O>* __restrict__ head, Xyzz<O>* __restrict__ tail) {
     const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t T = off[nbk];
-    const uint32_t j0 = l * (uint32_t)K;
-    if (j0 >= T) return;
-    const uint32_t j1 = min(j0 + (uint32_t)K, T);
+    if (l >= lanes) return;
+    const MsmDesc& m = desc[GM_FIND(desc, count, lane_base, l)];
+    const uint32_t e0 = off[m.bkt_base], e1 = off[m.bkt_base + m.nw * m.B];
+    const uint32_t j0 = e0 + (l - m.lane_base) * m.chunk;
+    if (j0 >= e1) return;
+    const uint32_t j1 = min(j0 + m.chunk, e1);
     Xyzz<O> acc;
     xyzz_set_inf(acc);
     uint32_t cur = (uint32_t)(entries[j0] >> 32);
@@ -216,12 +220,28 @@ __global__ void __launch_bounds__(64) k_accumulate(const Aff<O>* __restrict__ ba
         st16(&buckets[cur], acc);
 }
 
+// First and last accumulation lane whose chunk holds entries of bucket g
+// (entries [s, e) of the MSM described by m).
+__device__ __forceinline__ void bucket_lanes(const MsmDesc& m, const uint32_t* __restrict__ off, uint32_t s,
+                                             uint32_t e, uint32_t& l0, uint32_t& l1) {
+    const uint32_t e0 = off[m.bkt_base];
+    l0 = m.lane_base + (s - e0) / m.chunk;
+    l1 = m.lane_base + (e - 1 - e0) / m.chunk;
+}
+
 // One lane per bucket: empty buckets become O; a bucket cut across chunks is
-// tail[first lane] + head[every later lane it reaches].
+// tail[first lane] + head[every later lane it reaches]. A bucket cut across
+// more than HEAVY_SPAN chunks (few, very full buckets: a top window with few
+// bits, or repeated scalars) is queued for k_bucket_fixup_heavy instead of
+// being summed serially by one lane.
+constexpr uint32_t HEAVY_SPAN = 8;
+
 template <class O>
-__global__ void __launch_bounds__(64) k_bucket_fixup(const uint32_t* __restrict__ off, int nbk, int K,
+__global__ void __launch_bounds__(64) k_bucket_fixup(const MsmDesc* __restrict__ desc, int count, int nbk,
+                                                     const uint32_t* __restrict__ off,
                                                      const Xyzz<O>* __restrict__ head,
-                                                     const Xyzz<O>* __restrict__ tail, Xyzz<O>* __restrict__ buckets) {
+                                                     const Xyzz<O>* __restrict__ tail, Xyzz<O>* __restrict__ buckets,
+                                                     uint32_t* __restrict__ heavy, uint32_t* __restrict__ heavy_count) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= nbk) return;
     const uint32_t s = off[g], e = off[g + 1];
@@ -231,8 +251,13 @@ __global__ void __launch_bounds__(64) k_bucket_fixup(const uint32_t* __restrict_
         st16(&buckets[g], acc);
         return;
     }
-    const uint32_t l0 = s / (uint32_t)K, l1 = (e - 1) / (uint32_t)K;
+    uint32_t l0, l1;
+    bucket_lanes(desc[GM_FIND(desc, count, bkt_base, (uint32_t)g)], off, s, e, l0, l1);
     if (l0 == l1) return;
+    if (l1 - l0 > HEAVY_SPAN) {
+        heavy[atomicAdd(heavy_count, 1u)] = (uint32_t)g;
+        return;
+    }
     ld16(acc, &tail[l0]);
     for (uint32_t l = l0 + 1; l <= l1; ++l) {
         Xyzz<O> v;
@@ -240,6 +265,49 @@ __global__ void __launch_bounds__(64) k_bucket_fixup(const uint32_t* __restrict_
         xyzz_add<O>(acc, acc, v);
     }
     st16(&buckets[g], acc);
+}
+
+// One workgroup per queued heavy bucket (grid-stride over the queue): strided
+// per-thread sums of its chunk partials, then a tree in LDS.
+template <class O, int THREADS>
+__global__ void __launch_bounds__(THREADS) k_bucket_fixup_heavy(const MsmDesc* __restrict__ desc, int count,
+                                                                const uint32_t* __restrict__ off,
+                                                                const Xyzz<O>* __restrict__ head,
+                                                                const Xyzz<O>* __restrict__ tail,
+                                                                Xyzz<O>* __restrict__ buckets,
+                                                                const uint32_t* __restrict__ heavy,
+                                                                const uint32_t* __restrict__ heavy_count) {
+    __shared__ Xyzz<O> sh[THREADS];
+    const uint32_t nheavy = *heavy_count;
+    const int tid = threadIdx.x;
+    for (uint32_t h = blockIdx.x; h < nheavy; h += gridDim.x) {
+        const uint32_t g = heavy[h];
+        uint32_t l0, l1;
+        bucket_lanes(desc[GM_FIND(desc, count, bkt_base, g)], off, off[g], off[g + 1], l0, l1);
+        const uint32_t parts = l1 - l0 + 1;  // part 0 = tail[l0], part i = head[l0 + i]
+        Xyzz<O> acc;
+        xyzz_set_inf(acc);
+        for (uint32_t i = tid; i < parts; i += THREADS) {
+            Xyzz<O> v;
+            if (i == 0)
+                ld16(v, &tail[l0]);
+            else
+                ld16(v, &head[l0 + i]);
+            xyzz_add<O>(acc, acc, v);
+        }
+        sh[tid] = acc;
+        __syncthreads();
+        for (int s = THREADS / 2; s > 0; s >>= 1) {
+            if (tid < s) {
+                Xyzz<O> a = sh[tid], b = sh[tid + s];
+                xyzz_add<O>(a, a, b);
+                sh[tid] = a;
+            }
+            __syncthreads();
+        }
+        if (tid == 0) st16(&buckets[g], sh[0]);
+        __syncthreads();
+    }
 }
 
 // ----------------------------------------------------------- bucket reduce

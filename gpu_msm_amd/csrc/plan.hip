@@ -4,6 +4,8 @@
 
 #include <string.h>
 
+#include <algorithm>
+
 #include "engine.hpp"
 
 namespace gm {
@@ -94,8 +96,25 @@ bool make_plan(Plan& p, MsmDesc* d, int curve, int L, int bits, const MsmSpec* s
     p.bkt_total = (uint32_t)bkt;
     p.seg_total = (uint32_t)seg;
     p.win_total = (uint32_t)win;
-    p.chunk = 32;
-    p.lanes = (int)((rank + p.chunk - 1) / p.chunk);
+    // Entries per accumulation lane, per MSM (chunks never straddle two
+    // MSMs). Parallelism asks for ~64k lanes over the whole batch (chunk 32
+    // for big runs, down to 2 for small ones, so the serial depth of one mixed
+    // add per entry stays short); the fix-up asks that a typical bucket of the
+    // MSM spans at most ~4 chunks, so chunk >= the MSM's average bucket / 4.
+    int k_par = 32;
+    while (k_par > 2 && rank < (uint64_t)k_par * 65536) k_par >>= 1;
+    uint64_t lanes = 0;
+    for (int j = 0; j < count; ++j) {
+        MsmDesc& m = d[j];
+        const uint64_t avg = (uint64_t)m.n / m.B;
+        uint32_t k_span = 2;
+        while (k_span < 1024 && (uint64_t)k_span * 4 < avg) k_span <<= 1;
+        m.chunk = std::max<uint32_t>((uint32_t)k_par, k_span);
+        m.lane_base = (uint32_t)lanes;
+        lanes += ((uint64_t)m.nw * m.n + m.chunk - 1) / m.chunk;
+    }
+    if (lanes >= 0x7FFFFFFFull) return false;
+    p.lanes = (int)lanes;
     size_t cub = 0;
     if (hipcub::DeviceScan::ExclusiveSum(nullptr, cub, (uint32_t*)nullptr, (uint32_t*)nullptr, (int)(bkt + 1)) !=
         hipSuccess) {
@@ -119,10 +138,11 @@ bool make_plan(Plan& p, MsmDesc* d, int curve, int L, int bits, const MsmSpec* s
     p.o_buckets = take(bkt * xb);
     p.o_head = take((size_t)p.lanes * xb);
     p.o_tail = take((size_t)p.lanes * xb);
+    p.o_heavy = take((bkt + 1) * 4);
     p.o_segs = take(seg * xb);
     p.o_wsum = take(win * xb);
     p.o_out = take((size_t)count * aff_bytes(curve));
-    p.o_err = take(4);
+    p.o_err = take(8);  // error flag, heavy-bucket count
     p.o_cub = take(cub);
     p.total = o;
     return true;

@@ -28,7 +28,7 @@ hipError_t launch_sort(hipStream_t s, const Plan& p, const uint64_t* d_scalars, 
     // flag are cleared by a fill kernel, not hipMemsetAsync, so the sequence
     // can be captured in a hipGraph and replayed.
     GM_TRY(launch_fill_u32(s, cnt, nbk + 1, 0));
-    GM_TRY(launch_fill_u32(s, err, 1, 0));
+    GM_TRY(launch_fill_u32(s, err, 2, 0));  // error flag and heavy-bucket count
     if (p.n_total > 0) {
         k_digits_hist<<<blocks_for(p.n_total, 256), 256, 0, s>>>(desc, p.count, p.n_total, d_scalars, p.L, cnt, rank,
                                                                  err);

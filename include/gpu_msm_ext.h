@@ -83,20 +83,41 @@ int gm_msm_batch(void* results_affine, const gm_bases* b, const gm_scalars* s, c
                  int scalar_bits, int device_combine);
 
 /* A fixed batch bound to resident bases and scalars, with its own stream and
- * device workspace. use_graph != 0 captures the whole pipeline (sort,
- * accumulation, reduction, device window combine, result copy) in a hipGraph
- * at creation; every run replays it. The scalars' contents may change between
- * runs; the handles must outlive the batch. One batch must not be launched
- * again before gm_batch_wait returns; different batches may overlap. */
+ * device workspace. GM_BATCH_GRAPH captures the device pipeline (sort,
+ * accumulation, reduction, window sums and their copy to pinned host memory)
+ * in a hipGraph at creation; every launch replays it. gm_batch_wait then runs
+ * the window combines on host threads, or, with GM_BATCH_DEVICE_COMBINE, on
+ * the device inside the same graph (one lane per MSM). The scalars' contents
+ * may change between runs; the handles must outlive the batch. One batch must
+ * not be launched again before gm_batch_wait returns; different batches may
+ * overlap on the device. */
+#define GM_BATCH_GRAPH 0x1
+#define GM_BATCH_DEVICE_COMBINE 0x2
 typedef struct gm_batch gm_batch;
 int gm_batch_create(gm_batch** out, const gm_bases* b, const gm_scalars* s, const gm_msm_spec* specs, int count,
-                    int scalar_bits, int use_graph);
+                    int scalar_bits, int flags);
 int gm_batch_launch(gm_batch* bt);                       /* asynchronous */
 int gm_batch_wait(gm_batch* bt, void* results_affine);   /* count ABI affine records */
 int gm_batch_run(gm_batch* bt, void* results_affine);    /* launch + wait */
 int gm_batch_count(const gm_batch* bt);
 size_t gm_batch_workspace_bytes(const gm_batch* bt);
 void gm_batch_free(gm_batch* bt);
+
+/* ------------------------------------------------------------ planner MLP */
+
+/* FP32 MLP 3 -> 32 -> 32 (ReLU) -> 2 used by the MSM planner; parameters are
+ * W1[32][3] b1[32] W2[32][32] b2[32] W3[2][32] b3[2]. x: n x 3, y/t: n x 2.
+ * Loss L = (1/n) sum 0.5 (y - t)^2. */
+#define GM_MLP_PARAMS 1250
+int gm_mlp_forward(const float* params, const float* x, int n, float* y, uint32_t gpu_index);
+/* Full-batch gradient dL/dparams and L at `params`. */
+int gm_mlp_grad(const float* params, const float* x, const float* t, int n, float* grad, float* loss,
+                uint32_t gpu_index);
+/* `epochs` full-batch Adam steps (one step captured in a hipGraph and
+ * replayed); params updated in place; loss_curve (optional, epochs floats)
+ * receives the loss before each step. */
+int gm_mlp_train(float* params, const float* x, const float* t, int n, int epochs, float lr, float* loss_curve,
+                 uint32_t gpu_index);
 
 /* -------------------------------------------------------------- profiling */
 

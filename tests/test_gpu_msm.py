@@ -286,9 +286,11 @@ def test_batch_graph_replays_match_eager():
     eager = gm.msm_batch(b, s, specs, scalar_bits=299)
     bt = gm.Batch(b, s, specs, scalar_bits=299, use_graph=True)
     nog = gm.Batch(b, s, specs, scalar_bits=299, use_graph=False)
+    dev = gm.Batch(b, s, specs, scalar_bits=299, use_graph=True, device_combine=True)
     for rep in range(4):
         assert same(bt.run(), eager), rep
         assert same(nog.run(), eager), rep
+        assert same(dev.run(), eager), rep
         gm.msm_resident(b, s, n=100, scalar_bits=299)  # eager work between replays
     # launch/wait split and a second batch overlapping on the device
     bt.launch()
@@ -297,3 +299,26 @@ def test_batch_graph_replays_match_eager():
     assert same(nog.wait(), eager)
     bt.free()
     nog.free()
+    dev.free()
+
+
+def test_heavy_buckets_repeated_scalars_and_tiny_top_window():
+    """All scalars equal (every window's entries in one bucket) and a window
+    width whose top window has few bits: the chunk partials of those buckets
+    go through the parallel heavy-bucket fix-up. Closed form: s * sum(k_i) * G."""
+    n = 1 << 14
+    ks = o.gen_u64(97, n)
+    b = gm.Bases.from_k(gm.CURVE_G1, ks)
+    s_val = m.gen_scalars_modr(98, 1)[0]
+    sc = np.tile(o.int_to_limbs(s_val, 5), (n, 1))
+    s = gm.Scalars.upload(sc)
+    t = s_val * sum(int(k) for k in ks) % m.R_ORDER
+    want = m.ec_mul(m.FP_OPS, m.G1_STD, t)
+    for c in (0, 14, 16):
+        r = gm.msm_resident(b, s, scalar_bits=299, window=c)
+        assert as_point("g1", r[0]) == want, c
+    # tiny top window with random scalars, against the oracle on a subset
+    ss = o.gen_scalars_modr(99, n)
+    s2 = gm.Scalars.upload(ss)
+    full = gm.msm_resident(b, s2, scalar_bits=299, window=14)
+    assert same(full, gm.msm_resident(b, s2, scalar_bits=299, window=11))
