@@ -158,6 +158,25 @@ def planner_ab(pool, s, sizes, boff, soff, reference: np.ndarray, reps: int) -> 
     return out
 
 
+def bench_served(requests: int) -> dict:
+    """Continuous-batching server on the mixed stream (Poisson arrivals,
+    80% G1 / 20% G2, N log-uniform in [2^10, 2^20], bases resident): a
+    moderate rate for latency, a saturating rate for throughput, and the same
+    request list run plain (one MSM at a time)."""
+    from gpu_msm_amd import serve
+
+    st = serve.Stream()
+    out = {
+        "rate_200": serve.run(st, 200.0, requests, 10, 20, seed=5),
+        "saturation": serve.run(st, 5000.0, requests, 10, 20, seed=6),
+        "plain_same_requests": serve.plain(st, requests, 10, 20, seed=6),
+    }
+    out["saturation_vs_plain"] = round(out["saturation"]["msms_per_s"] / out["plain_same_requests"]["msms_per_s"], 2)
+    st.g1.free()
+    st.g2.free()
+    return out
+
+
 def cpu_baseline(bases_dev, scalars_np: np.ndarray, curve: str, n_full: int):
     """Oracle host Pippenger on a bounded sample (N = 2^16, window 16) on all
     host cores, scaled to N = 2^20 by the window-16 cost model
@@ -191,6 +210,8 @@ def main() -> None:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-g2", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--no-self-check", action="store_true",
                     help="skip the closed-form check (its n=1 MSM would mix into profiler averages)")
@@ -238,6 +259,12 @@ def main() -> None:
         print("bench: batched workload (256 MSMs, N log-uniform in [2^8, 2^18])", file=sys.stderr, flush=True)
         batched = bench_batch(args.batch_reps)
 
+    served = None
+    if not args.no_serve:
+        print("bench: served mixed stream (80% G1 / 20% G2, N log-uniform in [2^10, 2^20])", file=sys.stderr,
+              flush=True)
+        served = bench_served(args.serve_requests)
+
     cpu = None
     if not args.no_cpu_baseline:
         print("bench: cpu baseline (oracle, bounded sample)", file=sys.stderr, flush=True)
@@ -276,6 +303,7 @@ def main() -> None:
             "mixed_adds_per_launch": info["entries"],
         },
         "batched": batched,
+        "served": served,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

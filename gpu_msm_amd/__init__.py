@@ -57,6 +57,8 @@ EXPORTS = [
     "gm_scalars_upload", "gm_scalars_free", "gm_msm_resident", "gm_msm_batch",
     "gm_batch_create", "gm_batch_launch", "gm_batch_wait", "gm_batch_run", "gm_batch_count",
     "gm_batch_workspace_bytes", "gm_batch_free", "gm_mlp_forward", "gm_mlp_grad", "gm_mlp_train",
+    "gm_server_create", "gm_server_submit", "gm_server_wait", "gm_server_get_stats", "gm_server_inject_failures",
+    "gm_server_destroy",
     "gm_set_profiling", "gm_profile_last", "gm_last_msm_info",
 ]
 
@@ -116,6 +118,12 @@ def lib() -> ctypes.CDLL:
         "gm_mlp_forward": ([vp, vp, i, vp, u32], i),
         "gm_mlp_grad": ([vp, vp, vp, i, vp, vp, u32], i),
         "gm_mlp_train": ([vp, vp, vp, i, i, ctypes.c_float, vp, u32], i),
+        "gm_server_create": ([ctypes.POINTER(vp), vp, vp, vp], i),
+        "gm_server_submit": ([vp, i, i, i, vp, i, i, u32, ctypes.POINTER(ctypes.c_uint64)], i),
+        "gm_server_wait": ([vp, ctypes.c_uint64, vp, ctypes.POINTER(ctypes.c_double), i], i),
+        "gm_server_get_stats": ([vp, vp], i),
+        "gm_server_inject_failures": ([vp, i], i),
+        "gm_server_destroy": ([vp], None),
         "gm_set_profiling": ([u32, i], i),
         "gm_profile_last": ([u32, ctypes.POINTER(ctypes.c_float), i], i),
         "gm_last_msm_info": ([u32, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(ctypes.c_longlong)], i),
@@ -446,3 +454,81 @@ def last_msm_info(gpu_index: int = 0) -> dict:
     c, nw, e = ctypes.c_int(), ctypes.c_int(), ctypes.c_longlong()
     _check(lib().gm_last_msm_info(gpu_index, ctypes.byref(c), ctypes.byref(nw), ctypes.byref(e)), "msm info")
     return {"window": c.value, "windows": nw.value, "entries": e.value}
+
+
+# ------------------------------------------------------------------ server
+
+class _ServerConfig(ctypes.Structure):
+    _fields_ = [("gpu_index", ctypes.c_uint32), ("slots", ctypes.c_int), ("max_batch", ctypes.c_int),
+                ("hbm_budget_bytes", ctypes.c_size_t), ("planner", ctypes.c_int),
+                ("mlp_params", ctypes.c_void_p)]
+
+
+class _ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("failed", ctypes.c_uint64),
+                ("batches", ctypes.c_uint64), ("mean_batch", ctypes.c_double), ("p50_ms", ctypes.c_double),
+                ("p99_ms", ctypes.c_double), ("max_ms", ctypes.c_double), ("peak_inflight_bytes", ctypes.c_size_t),
+                ("peak_concurrent_batches", ctypes.c_int)]
+
+
+PLANNER_HEURISTIC = 0
+PLANNER_MLP = 1
+SUBMIT_BORROW = 0x1
+
+
+class Server:
+    """Continuous-batching MSM request loop over resident G1/G2 base pools
+    (gm_server_* in the library)."""
+
+    def __init__(self, g1_pool: Bases | None, g2_pool: Bases | None, slots: int = 3, max_batch: int = 64,
+                 hbm_budget_bytes: int = 32 << 30, planner: int = PLANNER_HEURISTIC, mlp_params=None,
+                 gpu_index: int = 0):
+        self.pools = (g1_pool, g2_pool)
+        self._mlp = None if mlp_params is None else np.ascontiguousarray(mlp_params, dtype=np.float32)
+        cfg = _ServerConfig(gpu_index, slots, max_batch, hbm_budget_bytes, planner,
+                            None if self._mlp is None else self._mlp.ctypes.data)
+        h = ctypes.c_void_p()
+        _check(lib().gm_server_create(ctypes.byref(h), ctypes.byref(cfg), g1_pool.h if g1_pool else None,
+                                      g2_pool.h if g2_pool else None), "server create")
+        self.h = h.value
+        self._borrowed = {}
+
+    def submit(self, curve: int, base_offset: int, scalars: np.ndarray, scalar_bits: int = 0,
+               borrow: bool = False) -> int:
+        sc = np.ascontiguousarray(scalars, dtype=np.uint64)
+        if sc.ndim == 1:
+            sc = sc.reshape(-1, 1)
+        t = ctypes.c_uint64()
+        _check(lib().gm_server_submit(self.h, curve, base_offset, sc.shape[0], _ptr(sc), sc.shape[1], scalar_bits,
+                                      SUBMIT_BORROW if borrow else 0, ctypes.byref(t)), "submit")
+        if borrow:
+            self._borrowed[t.value] = sc  # keep alive until waited
+        return t.value
+
+    def wait(self, ticket: int, curve: int, timeout_ms: int = -1):
+        """Returns (rc, affine record or None, latency ms)."""
+        out = np.zeros(1, dtype=G1_DTYPE if curve == CURVE_G1 else G2_DTYPE)
+        lat = ctypes.c_double()
+        rc = lib().gm_server_wait(self.h, ticket, _ptr(out), ctypes.byref(lat), timeout_ms)
+        if rc != -6:
+            self._borrowed.pop(ticket, None)
+        return rc, (out if rc == 0 else None), lat.value
+
+    def stats(self) -> dict:
+        st = _ServerStats()
+        _check(lib().gm_server_get_stats(self.h, ctypes.byref(st)), "server stats")
+        return {k: getattr(st, k) for k, _ in _ServerStats._fields_}
+
+    def inject_failures(self, batches: int) -> None:
+        _check(lib().gm_server_inject_failures(self.h, batches), "inject failures")
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            lib().gm_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -17,6 +17,7 @@
 #include "../../include/rust_wrapper.h"
 #include "engine.hpp"
 #include "host_field.hpp"
+#include "resident.hpp"
 
 #define GM_TRY_HOST(x)                   \
     do {                                 \
@@ -304,19 +305,6 @@ int msm_host_ptrs(void* result_proj, const void* points, const uint64_t* scalars
 
 // ------------------------------------------------------------ resident data
 
-struct gm_bases {
-    int curve;
-    int n;
-    int dev;
-    void* d;  // Aff<O>[n]
-};
-
-struct gm_scalars {
-    int n;
-    int limbs;
-    int dev;
-    uint64_t* d;
-};
 
 namespace {
 

@@ -87,7 +87,10 @@ size_t aff_bytes(int curve);
 int choose_window(int curve, int n, int bits);
 int choose_seg(int B);
 // Fills p (and p.host_desc[0..count)) for a batch; false on invalid specs.
-bool make_plan(Plan& p, MsmDesc* desc_storage, int curve, int L, int bits, const MsmSpec* specs, int count);
+// exact_scan_bytes = false bounds the scan's temporary storage instead of
+// asking the device (fast, for admission estimates).
+bool make_plan(Plan& p, MsmDesc* desc_storage, int curve, int L, int bits, const MsmSpec* specs, int count,
+               bool exact_scan_bytes = true);
 
 template <class O>
 struct Engine {

@@ -49,7 +49,8 @@ int choose_window(int curve, int n, int bits) {
     return best;
 }
 
-bool make_plan(Plan& p, MsmDesc* d, int curve, int L, int bits, const MsmSpec* specs, int count) {
+bool make_plan(Plan& p, MsmDesc* d, int curve, int L, int bits, const MsmSpec* specs, int count,
+               bool exact_scan_bytes) {
     memset(&p, 0, sizeof p);
     if (count < 1 || L < 1 || L > 8 || !d || !specs) return false;
     if (bits <= 0 || bits > 64 * L) bits = 64 * L;
@@ -116,8 +117,9 @@ bool make_plan(Plan& p, MsmDesc* d, int curve, int L, int bits, const MsmSpec* s
     if (lanes >= 0x7FFFFFFFull) return false;
     p.lanes = (int)lanes;
     size_t cub = 0;
-    if (hipcub::DeviceScan::ExclusiveSum(nullptr, cub, (uint32_t*)nullptr, (uint32_t*)nullptr, (int)(bkt + 1)) !=
-        hipSuccess) {
+    if (!exact_scan_bytes ||
+        hipcub::DeviceScan::ExclusiveSum(nullptr, cub, (uint32_t*)nullptr, (uint32_t*)nullptr, (int)(bkt + 1)) !=
+            hipSuccess) {
         // No device to ask (plan queries on a host without a GPU): a bound on
         // the decoupled-lookback scan state.
         cub = (bkt / 128 + 1024) * 16 + 65536;

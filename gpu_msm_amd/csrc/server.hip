@@ -1,0 +1,574 @@
+// Served MSM request loop: continuous batching of mixed G1/G2 requests over
+// bases resident in HBM.
+//
+// Requests (curve, base offset, n, scalars) are queued by gm_server_submit.
+// A scheduler thread forms a batch from the head of the queue (one curve and
+// scalar width per batch, FIFO order) as soon as a stream slot is free,
+// admitting requests while the batch's device footprint (pipeline workspace +
+// scalars) fits what is left of the HBM budget, then launches the whole
+// pipeline on that slot's stream. Several slots run at once, so batches that
+// form while others are on the GPU overlap on the device. A completion thread
+// waits for batches in launch order, runs the window combines on host threads
+// while the next batches keep the GPU busy, and publishes each request's
+// affine result and latency (submit -> result ready).
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/gpu_msm_ext.h"
+#include "engine.hpp"
+#include "host_field.hpp"
+#include "resident.hpp"
+
+#define GM_TRY_S(x)                      \
+    do {                                 \
+        hipError_t e_ = (x);             \
+        if (e_ != hipSuccess) return e_; \
+    } while (0)
+
+namespace {
+
+using gm::Aff;
+using gm::Fp2Ops;
+using gm::FpOps;
+using gm::Xyzz;
+
+double now_ms() {
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+struct Request {
+    uint64_t ticket = 0;
+    int curve = 0, base_off = 0, n = 0, limbs = 0, bits = 0;
+    std::vector<uint64_t> scalars;
+    const uint64_t* borrowed = nullptr;  // caller-owned scalars (GM_SUBMIT_BORROW)
+    double t_submit = 0, t_done = 0;
+    bool done = false;
+    int rc = 0;
+    uint8_t result[232];
+};
+
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t ev = nullptr;
+    uint8_t* ws = nullptr;
+    size_t ws_cap = 0;
+    uint8_t* d_sc = nullptr;
+    size_t sc_cap = 0;
+    uint8_t* pinned = nullptr;
+    size_t pinned_cap = 0;
+    bool busy = false;
+    bool fail = false;
+    std::vector<Request*> reqs;
+    std::vector<gm::MsmDesc> desc;
+    gm::Plan p{};
+    int curve = 0;
+    size_t footprint = 0;
+    size_t res_bytes = 0;
+};
+
+// ---- host side of the window combine (same arithmetic as capi.hip)
+
+void elem_from_dev(gmh::Fp& r, const uint32_t* v) {
+    gmh::Fp t;
+    gmh::from_limbs28(t, v);
+    gmh::reduce(r, t);
+}
+void elem_from_dev(gmh::Fp2& r, const uint32_t* v) {
+    elem_from_dev(r.c0, v);
+    elem_from_dev(r.c1, v + 16);
+}
+
+template <class HF>
+bool combine(HF& x, HF& y, const uint32_t* raw, int nw, int c) {
+    constexpr int W = sizeof(HF) == sizeof(gmh::Fp) ? 16 : 32;
+    auto load = [&](gmh::Xyzz<HF>& p, int w) {
+        const uint32_t* b = raw + (size_t)w * 4 * W;
+        elem_from_dev(p.X, b);
+        elem_from_dev(p.Y, b + W);
+        elem_from_dev(p.ZZ, b + 2 * W);
+        elem_from_dev(p.ZZZ, b + 3 * W);
+    };
+    gmh::Xyzz<HF> r, v;
+    load(r, nw - 1);
+    for (int w = nw - 2; w >= 0; --w) {
+        for (int k = 0; k < c; ++k) gmh::xdbl(r, r);
+        load(v, w);
+        gmh::xadd(r, r, v);
+    }
+    return gmh::to_affine(x, y, r);
+}
+
+template <class HF>
+void write_affine(uint8_t* out, bool finite, const HF& x, const HF& y) {
+    constexpr int E = sizeof(HF) / 8;
+    uint64_t buf[2 * E + 1];
+    memset(buf, 0, sizeof buf);
+    if (finite) {
+        memcpy(buf, &x, sizeof(HF));
+        memcpy(buf + E, &y, sizeof(HF));
+    } else {
+        buf[2 * E] = 1;
+    }
+    memcpy(out, buf, sizeof buf);
+}
+
+// ---- planner MLP (host inference; same network as csrc/mlp.hip)
+
+struct HostMlp {
+    std::vector<float> P;
+    bool ok() const { return P.size() == GM_MLP_PARAMS; }
+    // (c, seg) for one MSM, or (0, 0) to use the heuristic
+    void plan(int curve, int n, int batch, int& c, int& seg) const {
+        c = 0;
+        seg = 0;
+        if (!ok()) return;
+        const float x[3] = {(float)((log2((double)std::max(n, 1)) - 13.0) / 5.0),
+                            (float)((log2((double)std::max(batch, 1)) - 3.0) / 3.0), (float)(curve - 1)};
+        float h1[32], h2[32], y[2];
+        const float *W1 = P.data(), *b1 = W1 + 96, *W2 = b1 + 32, *b2 = W2 + 1024, *W3 = b2 + 32, *b3 = W3 + 64;
+        for (int j = 0; j < 32; ++j) {
+            float a = b1[j];
+            for (int i = 0; i < 3; ++i) a = fmaf(W1[j * 3 + i], x[i], a);
+            h1[j] = a > 0.f ? a : 0.f;
+        }
+        for (int j = 0; j < 32; ++j) {
+            float a = b2[j];
+            for (int i = 0; i < 32; ++i) a = fmaf(W2[j * 32 + i], h1[i], a);
+            h2[j] = a > 0.f ? a : 0.f;
+        }
+        for (int k = 0; k < 2; ++k) {
+            float a = b3[k];
+            for (int i = 0; i < 32; ++i) a = fmaf(W3[k * 32 + i], h2[i], a);
+            y[k] = a;
+        }
+        const double cc = y[0] * 4.0 + 12.0, ls = y[1] * 1.5 + 3.0;
+        if (!std::isfinite(cc) || !std::isfinite(ls) || cc < 2.0 || cc > 21.0) return;  // heuristic fallback
+        c = std::min(20, std::max(3, (int)lround(cc)));
+        seg = 1 << (int)lround(std::min(5.0, std::max(1.0, ls)));
+        seg = std::min(seg, 1 << (c - 1));
+    }
+};
+
+}  // namespace
+
+struct gm_server {
+    gm_server_config cfg{};
+    const gm_bases* pool[3] = {nullptr, nullptr, nullptr};
+    HostMlp mlp;
+    std::mutex mu;
+    std::condition_variable cv_sched, cv_done, cv_comp;
+    std::deque<Request*> queue;
+    std::unordered_map<uint64_t, Request*> table;
+    std::vector<Slot> slots;
+    std::deque<int> launched;  // slot ids in launch order
+    size_t inflight_bytes = 0;
+    uint64_t next_ticket = 1;
+    bool stop = false;
+    int fail_batches = 0;
+    // stats
+    uint64_t submitted = 0, completed = 0, batches = 0, failed = 0;
+    uint64_t batched_msms = 0;
+    size_t peak_inflight = 0;
+    int peak_busy = 0;
+    std::vector<double> lat;
+    std::thread th_sched, th_comp;
+};
+
+namespace {
+
+size_t footprint_of(const gm::Plan& p, const std::vector<Request*>& rs) {
+    size_t sc = 0;
+    for (auto* r : rs) sc += (size_t)r->n * r->limbs * 8;
+    return p.total + gm::al_size(sc);
+}
+
+bool plan_batch(gm_server* S, const std::vector<Request*>& rs, gm::Plan& p, std::vector<gm::MsmDesc>& desc,
+                bool exact = true) {
+    std::vector<gm::MsmSpec> sp(rs.size());
+    int so = 0;
+    for (size_t k = 0; k < rs.size(); ++k) {
+        int c = 0, seg = 0;
+        if (S->cfg.planner == GM_PLANNER_MLP) S->mlp.plan(rs[k]->curve, rs[k]->n, (int)rs.size(), c, seg);
+        sp[k] = {rs[k]->n, rs[k]->base_off, so, c, seg};
+        so += rs[k]->n;
+    }
+    desc.resize(rs.size());
+    return gm::make_plan(p, desc.data(), rs[0]->curve, rs[0]->limbs, rs[0]->bits, sp.data(), (int)rs.size(), exact);
+}
+
+hipError_t grow(uint8_t*& ptr, size_t& cap, size_t want, bool pinned) {
+    if (want <= cap) return hipSuccess;
+    if (ptr) {
+        if (pinned)
+            hipHostFree(ptr);
+        else
+            hipFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+    }
+    want += want / 4;
+    hipError_t e = pinned ? hipHostMalloc((void**)&ptr, want, hipHostMallocDefault) : hipMalloc(&ptr, want);
+    if (e == hipSuccess) cap = want;
+    return e;
+}
+
+template <class O>
+hipError_t launch_slot(gm_server* S, Slot& sl) {
+    const gm_bases* b = S->pool[sl.curve];
+    uint8_t* sc = sl.d_sc;
+    for (auto* r : sl.reqs) {
+        const size_t bytes = (size_t)r->n * r->limbs * 8;
+        GM_TRY_S(hipMemcpyAsync(sc, r->borrowed ? r->borrowed : r->scalars.data(), bytes, hipMemcpyHostToDevice,
+                                sl.stream));
+        sc += bytes;
+    }
+    GM_TRY_S(hipMemcpyAsync(sl.ws + sl.p.o_desc, sl.pinned + sl.res_bytes + 256, sl.desc.size() * sizeof(gm::MsmDesc),
+                            hipMemcpyHostToDevice, sl.stream));
+    GM_TRY_S(gm::Engine<O>::run(sl.stream, sl.p, reinterpret_cast<const Aff<O>*>(b->d),
+                                reinterpret_cast<const uint64_t*>(sl.d_sc), sl.ws, nullptr));
+    GM_TRY_S(hipMemcpyAsync(sl.pinned, sl.ws + sl.p.o_wsum, sl.res_bytes, hipMemcpyDeviceToHost, sl.stream));
+    GM_TRY_S(hipMemcpyAsync(sl.pinned + sl.res_bytes, sl.ws + sl.p.o_err, 4, hipMemcpyDeviceToHost, sl.stream));
+    return hipEventRecord(sl.ev, sl.stream);
+}
+
+void scheduler(gm_server* S) {
+    hipSetDevice((int)S->cfg.gpu_index);
+    for (;;) {
+        std::unique_lock<std::mutex> lk(S->mu);
+        int sid = -1;
+        S->cv_sched.wait(lk, [&] {
+            if (S->stop) return true;
+            if (S->queue.empty()) return false;
+            for (size_t i = 0; i < S->slots.size(); ++i)
+                if (!S->slots[i].busy) return true;
+            return false;
+        });
+        if (S->stop) return;
+        for (size_t i = 0; i < S->slots.size(); ++i)
+            if (!S->slots[i].busy) {
+                sid = (int)i;
+                break;
+            }
+        // admission: FIFO over one (curve, limbs, bits) class, by footprint
+        // (estimated plans during admission, the exact plan once at the end)
+        std::vector<Request*> take;
+        gm::Plan p{};
+        std::vector<gm::MsmDesc> desc;
+        const Request* head = S->queue.front();
+        const size_t budget = S->cfg.hbm_budget_bytes;
+        std::vector<size_t> qidx;
+        Request* reject = nullptr;
+        size_t reject_qi = 0;
+        for (size_t qi = 0; qi < S->queue.size() && (int)take.size() < S->cfg.max_batch; ++qi) {
+            Request* r = S->queue[qi];
+            if (r->curve != head->curve || r->limbs != head->limbs || r->bits != head->bits) continue;
+            take.push_back(r);
+            gm::Plan tp{};
+            std::vector<gm::MsmDesc> td;
+            const bool ok = plan_batch(S, take, tp, td, false);
+            if (!ok) {
+                take.pop_back();
+                if (take.empty()) {
+                    reject = r;  // cannot be planned at all
+                    reject_qi = qi;
+                }
+                break;
+            }
+            if (S->inflight_bytes + footprint_of(tp, take) > budget) {
+                take.pop_back();
+                if (take.empty() && S->inflight_bytes == 0) {
+                    take.push_back(r);  // larger than the budget: runs alone on an idle device
+                    qidx.push_back(qi);
+                }
+                break;
+            }
+            qidx.push_back(qi);
+        }
+        if (reject) {
+            S->queue.erase(S->queue.begin() + (long)reject_qi);
+            reject->rc = -1;
+            reject->done = true;
+            reject->t_done = now_ms();
+            S->completed += 1;
+            S->failed += 1;
+            S->cv_done.notify_all();
+            continue;
+        }
+        if (take.empty()) {
+            // wait for an in-flight batch to release HBM
+            S->cv_sched.wait_for(lk, std::chrono::milliseconds(2));
+            continue;
+        }
+        if (!plan_batch(S, take, p, desc, true)) {
+            S->cv_sched.wait_for(lk, std::chrono::milliseconds(2));
+            continue;
+        }
+        for (auto it = qidx.rbegin(); it != qidx.rend(); ++it) S->queue.erase(S->queue.begin() + (long)*it);
+        Slot& sl = S->slots[sid];
+        sl.busy = true;
+        sl.reqs = take;
+        sl.p = p;
+        sl.desc = desc;
+        sl.curve = head->curve;
+        sl.footprint = footprint_of(p, take);
+        sl.fail = S->fail_batches > 0;
+        if (sl.fail) --S->fail_batches;
+        S->inflight_bytes += sl.footprint;
+        S->peak_inflight = std::max(S->peak_inflight, S->inflight_bytes);
+        int busy = 0;
+        for (auto& s2 : S->slots) busy += s2.busy;
+        S->peak_busy = std::max(S->peak_busy, busy);
+        lk.unlock();
+
+        // device work outside the lock
+        sl.res_bytes = (size_t)p.win_total * gm::xyzz_bytes(sl.curve);
+        size_t sc_bytes = 0;
+        for (auto* r : take) sc_bytes += (size_t)r->n * r->limbs * 8;
+        hipError_t e = grow(sl.ws, sl.ws_cap, p.total, false);
+        if (e == hipSuccess) e = grow(sl.d_sc, sl.sc_cap, sc_bytes, false);
+        if (e == hipSuccess) e = grow(sl.pinned, sl.pinned_cap, sl.res_bytes + 256 + desc.size() * sizeof(gm::MsmDesc),
+                                      true);
+        if (e == hipSuccess) {
+            memcpy(sl.pinned + sl.res_bytes + 256, sl.desc.data(), sl.desc.size() * sizeof(gm::MsmDesc));
+            if (sl.fail)
+                e = hipEventRecord(sl.ev, sl.stream);  // injected failure: nothing is launched
+            else
+                e = sl.curve == GM_CURVE_G1 ? launch_slot<FpOps>(S, sl) : launch_slot<Fp2Ops>(S, sl);
+        }
+        if (e != hipSuccess) sl.fail = true;
+        lk.lock();
+        S->launched.push_back(sid);
+        S->batches += 1;
+        S->batched_msms += take.size();
+        S->cv_comp.notify_one();
+    }
+}
+
+void completer(gm_server* S) {
+    hipSetDevice((int)S->cfg.gpu_index);
+    for (;;) {
+        int sid;
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            S->cv_comp.wait(lk, [&] { return S->stop || !S->launched.empty(); });
+            if (S->launched.empty()) return;  // stop with nothing in flight
+            sid = S->launched.front();
+        }
+        Slot& sl = S->slots[sid];
+        hipError_t e = hipEventSynchronize(sl.ev);
+        int rc = 0;
+        if (sl.fail || e != hipSuccess) {
+            rc = -4;
+        } else {
+            uint32_t err;
+            memcpy(&err, sl.pinned + sl.res_bytes, 4);
+            if (err) rc = -5;
+        }
+        const int count = (int)sl.reqs.size();
+        if (rc == 0) {
+            const uint32_t* raw = reinterpret_cast<const uint32_t*>(sl.pinned);
+            auto work = [&](int j0, int j1) {
+                for (int j = j0; j < j1; ++j) {
+                    const gm::MsmDesc& d = sl.desc[j];
+                    if (sl.curve == GM_CURVE_G1) {
+                        gmh::Fp x, y;
+                        const bool f = combine<gmh::Fp>(x, y, raw + (size_t)d.win_base * 64, (int)d.nw, (int)d.c);
+                        write_affine<gmh::Fp>(sl.reqs[j]->result, f, x, y);
+                    } else {
+                        gmh::Fp2 x, y;
+                        const bool f = combine<gmh::Fp2>(x, y, raw + (size_t)d.win_base * 128, (int)d.nw, (int)d.c);
+                        write_affine<gmh::Fp2>(sl.reqs[j]->result, f, x, y);
+                    }
+                }
+            };
+            const int nt = count < 4 ? 1 : std::min(count, 8);
+            if (nt == 1) {
+                work(0, count);
+            } else {
+                std::vector<std::thread> th;
+                const int per = (count + nt - 1) / nt;
+                for (int t = 0; t < nt; ++t) {
+                    const int j0 = t * per, j1 = std::min(count, j0 + per);
+                    if (j0 < j1) th.emplace_back(work, j0, j1);
+                }
+                for (auto& t : th) t.join();
+            }
+        }
+        const double t = now_ms();
+        std::lock_guard<std::mutex> lk(S->mu);
+        for (auto* r : sl.reqs) {
+            r->rc = rc;
+            r->done = true;
+            r->t_done = t;
+            r->scalars.clear();
+            r->scalars.shrink_to_fit();
+            S->completed += 1;
+            if (rc == 0)
+                S->lat.push_back(t - r->t_submit);
+            else
+                S->failed += 1;
+        }
+        sl.reqs.clear();
+        sl.busy = false;
+        S->inflight_bytes -= sl.footprint;
+        S->launched.pop_front();
+        S->cv_done.notify_all();
+        S->cv_sched.notify_one();
+    }
+}
+
+double percentile(std::vector<double> v, double q) {
+    if (v.empty()) return 0.0;
+    std::sort(v.begin(), v.end());
+    const double pos = q * (double)(v.size() - 1);
+    const size_t i = (size_t)pos;
+    const double f = pos - (double)i;
+    return i + 1 < v.size() ? v[i] * (1 - f) + v[i + 1] * f : v[i];
+}
+
+}  // namespace
+
+extern "C" {
+
+int gm_server_create(gm_server** out, const gm_server_config* cfg, const gm_bases* g1_pool, const gm_bases* g2_pool) {
+    if (!out || !cfg || (!g1_pool && !g2_pool)) return -1;
+    if (cfg->slots < 1 || cfg->slots > 16 || cfg->max_batch < 1) return -1;
+    if ((g1_pool && g1_pool->curve != GM_CURVE_G1) || (g2_pool && g2_pool->curve != GM_CURVE_G2)) return -1;
+    if (hipSetDevice((int)cfg->gpu_index) != hipSuccess) return -2;
+    gm_server* S = new gm_server();
+    S->cfg = *cfg;
+    S->pool[GM_CURVE_G1] = g1_pool;
+    S->pool[GM_CURVE_G2] = g2_pool;
+    if (cfg->planner == GM_PLANNER_MLP) {
+        if (!cfg->mlp_params) {
+            delete S;
+            return -1;
+        }
+        S->mlp.P.assign(cfg->mlp_params, cfg->mlp_params + GM_MLP_PARAMS);
+    }
+    S->slots.resize((size_t)cfg->slots);
+    for (auto& sl : S->slots) {
+        if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming) != hipSuccess) {
+            gm_server_destroy(S);
+            return -2;
+        }
+    }
+    S->th_sched = std::thread(scheduler, S);
+    S->th_comp = std::thread(completer, S);
+    *out = S;
+    return 0;
+}
+
+int gm_server_submit(gm_server* S, int curve, int base_offset, int n, const uint64_t* scalars, int limbs,
+                     int scalar_bits, uint32_t flags, uint64_t* ticket) {
+    const double t0 = now_ms();
+    if (!S || !scalars || !ticket || n <= 0 || limbs < 1 || limbs > 8 || base_offset < 0) return -1;
+    if (curve != GM_CURVE_G1 && curve != GM_CURVE_G2) return -1;
+    const gm_bases* b = S->pool[curve];
+    if (!b || (long)base_offset + n > b->n) return -1;
+    Request* r = new Request();
+    r->curve = curve;
+    r->base_off = base_offset;
+    r->n = n;
+    r->limbs = limbs;
+    r->bits = scalar_bits;
+    if (flags & GM_SUBMIT_BORROW)
+        r->borrowed = scalars;
+    else
+        r->scalars.assign(scalars, scalars + (size_t)n * limbs);
+    r->t_submit = t0;
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (S->stop) {
+        delete r;
+        return -1;
+    }
+    r->ticket = S->next_ticket++;
+    S->table[r->ticket] = r;
+    S->queue.push_back(r);
+    S->submitted += 1;
+    *ticket = r->ticket;
+    S->cv_sched.notify_one();
+    return 0;
+}
+
+int gm_server_wait(gm_server* S, uint64_t ticket, void* result_affine, double* latency_ms, int timeout_ms) {
+    if (!S || !result_affine) return -1;
+    std::unique_lock<std::mutex> lk(S->mu);
+    auto it = S->table.find(ticket);
+    if (it == S->table.end()) return -1;
+    Request* r = it->second;
+    auto pred = [&] { return r->done; };
+    if (timeout_ms < 0)
+        S->cv_done.wait(lk, pred);
+    else if (!S->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), pred))
+        return -6;
+    const int rc = r->rc;
+    if (rc == 0) memcpy(result_affine, r->result, r->curve == GM_CURVE_G1 ? 120 : 232);
+    if (latency_ms) *latency_ms = r->t_done - r->t_submit;
+    S->table.erase(it);
+    delete r;
+    return rc;
+}
+
+int gm_server_get_stats(gm_server* S, gm_server_stats* out) {
+    if (!S || !out) return -1;
+    std::lock_guard<std::mutex> lk(S->mu);
+    memset(out, 0, sizeof *out);
+    out->submitted = S->submitted;
+    out->completed = S->completed;
+    out->failed = S->failed;
+    out->batches = S->batches;
+    out->mean_batch = S->batches ? (double)S->batched_msms / (double)S->batches : 0.0;
+    out->p50_ms = percentile(S->lat, 0.50);
+    out->p99_ms = percentile(S->lat, 0.99);
+    out->max_ms = S->lat.empty() ? 0.0 : *std::max_element(S->lat.begin(), S->lat.end());
+    out->peak_inflight_bytes = S->peak_inflight;
+    out->peak_concurrent_batches = S->peak_busy;
+    return 0;
+}
+
+int gm_server_inject_failures(gm_server* S, int batches) {
+    if (!S || batches < 0) return -1;
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->fail_batches = batches;
+    return 0;
+}
+
+void gm_server_destroy(gm_server* S) {
+    if (!S) return;
+    {
+        std::unique_lock<std::mutex> lk(S->mu);
+        // drain: wait until queued and in-flight work has completed
+        S->cv_done.wait(lk, [&] { return S->queue.empty() && S->launched.empty(); });
+        S->stop = true;
+        S->cv_sched.notify_all();
+        S->cv_comp.notify_all();
+    }
+    if (S->th_sched.joinable()) S->th_sched.join();
+    if (S->th_comp.joinable()) S->th_comp.join();
+    hipSetDevice((int)S->cfg.gpu_index);
+    for (auto& sl : S->slots) {
+        if (sl.stream) hipStreamSynchronize(sl.stream);
+        if (sl.ev) hipEventDestroy(sl.ev);
+        if (sl.stream) hipStreamDestroy(sl.stream);
+        if (sl.ws) hipFree(sl.ws);
+        if (sl.d_sc) hipFree(sl.d_sc);
+        if (sl.pinned) hipHostFree(sl.pinned);
+    }
+    for (auto& kv : S->table) delete kv.second;
+    delete S;
+}
+
+}  // extern "C"

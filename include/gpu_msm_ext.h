@@ -119,6 +119,51 @@ int gm_mlp_grad(const float* params, const float* x, const float* t, int n, floa
 int gm_mlp_train(float* params, const float* x, const float* t, int n, int epochs, float lr, float* loss_curve,
                  uint32_t gpu_index);
 
+/* --------------------------------------------------------- request server */
+
+/* Continuous batching of mixed G1/G2 MSM requests over resident base pools.
+ * A scheduler thread forms batches from the head of the queue (one curve per
+ * batch) whenever one of `slots` stream slots is free, admitting requests
+ * while the batch's device footprint fits the HBM budget left by in-flight
+ * batches; batches on different slots overlap on the device; a completion
+ * thread finishes window combines on host threads. */
+#define GM_PLANNER_HEURISTIC 0
+#define GM_PLANNER_MLP 1
+typedef struct gm_server_config {
+    uint32_t gpu_index;
+    int slots;                /* batches in flight at once (1..16) */
+    int max_batch;            /* MSMs per batch */
+    size_t hbm_budget_bytes;  /* admission limit for in-flight footprints */
+    int planner;              /* GM_PLANNER_* */
+    const float* mlp_params;  /* GM_MLP_PARAMS floats when planner == MLP */
+} gm_server_config;
+
+typedef struct gm_server_stats {
+    uint64_t submitted, completed, failed, batches;
+    double mean_batch;           /* MSMs per batch */
+    double p50_ms, p99_ms, max_ms; /* submit -> result latency of completed requests */
+    size_t peak_inflight_bytes;
+    int peak_concurrent_batches;
+} gm_server_stats;
+
+typedef struct gm_server gm_server;
+int gm_server_create(gm_server** out, const gm_server_config* cfg, const gm_bases* g1_pool, const gm_bases* g2_pool);
+/* Queues sum_i scalars[i] * pool[base_offset + i]. The scalars are copied,
+ * or with GM_SUBMIT_BORROW read in place (the caller keeps them unchanged
+ * until gm_server_wait returns for this ticket). */
+#define GM_SUBMIT_BORROW 0x1
+int gm_server_submit(gm_server* s, int curve, int base_offset, int n, const uint64_t* scalars, int limbs,
+                     int scalar_bits, uint32_t flags, uint64_t* ticket);
+/* Waits for a ticket (timeout_ms < 0: forever); 0 and the affine ABI record
+ * on success, the request's error code otherwise, -6 on timeout. */
+int gm_server_wait(gm_server* s, uint64_t ticket, void* result_affine, double* latency_ms, int timeout_ms);
+int gm_server_get_stats(gm_server* s, gm_server_stats* out);
+/* Test hook: the next `batches` batches fail as if the device had failed
+ * (no kernel is launched); their requests complete with -4. */
+int gm_server_inject_failures(gm_server* s, int batches);
+/* Drains queued and in-flight requests, then stops. */
+void gm_server_destroy(gm_server* s);
+
 /* -------------------------------------------------------------- profiling */
 
 /* Record HIP events between pipeline stages of subsequent MSMs on this GPU. */
