@@ -16,6 +16,7 @@
 #include "../../include/gpu_msm_ext.h"
 #include "../../include/rust_wrapper.h"
 #include "engine.hpp"
+#include "host_combine.hpp"
 #include "host_field.hpp"
 #include "resident.hpp"
 
@@ -30,6 +31,15 @@ using gm::Fp2Ops;
 using gm::FpOps;
 using gm::Plan;
 using gm::Xyzz;
+using gmc::AffResult;
+using gmc::dev_elem_to_host;
+using gmc::get_elem;
+using gmc::HostOf;
+using gmc::put_elem;
+using gmc::words_of;
+using gmc::write_affine_abi;
+using gmc::write_proj_abi;
+using gmc::host_combine;
 
 // ------------------------------------------------------------------ context
 
@@ -111,101 +121,9 @@ hipError_t ensure_pinned(DevCtx* c, size_t bytes) {
     return hipSuccess;
 }
 
-// ----------------------------------------------------- host-side conversions
-
-void dev_elem_to_host(gmh::Fp& r, const uint32_t* v) {
-    gmh::Fp t;
-    gmh::from_limbs28(t, v);
-    gmh::reduce(r, t);
-}
-void dev_elem_to_host(gmh::Fp2& r, const uint32_t* v) {
-    dev_elem_to_host(r.c0, v);
-    dev_elem_to_host(r.c1, v + 16);
-}
 
 template <class HF>
-constexpr int words_of() {
-    return sizeof(HF) == sizeof(gmh::Fp) ? 16 : 32;
-}
-
-// Horner over nw XYZZ window sums (raw device words), most significant first.
-template <class HF>
-bool host_combine(HF& x, HF& y, const uint32_t* raw, int nw, int c) {
-    constexpr int W = words_of<HF>();
-    auto load = [&](gmh::Xyzz<HF>& p, int w) {
-        const uint32_t* b = raw + (size_t)w * 4 * W;
-        dev_elem_to_host(p.X, b);
-        dev_elem_to_host(p.Y, b + W);
-        dev_elem_to_host(p.ZZ, b + 2 * W);
-        dev_elem_to_host(p.ZZZ, b + 3 * W);
-    };
-    gmh::Xyzz<HF> r, v;
-    load(r, nw - 1);
-    for (int w = nw - 2; w >= 0; --w) {
-        for (int k = 0; k < c; ++k) gmh::xdbl(r, r);
-        load(v, w);
-        gmh::xadd(r, r, v);
-    }
-    return gmh::to_affine(x, y, r);
-}
-
-void put_elem(uint64_t* w, const gmh::Fp& a) { memcpy(w, a.l, 56); }
-void put_elem(uint64_t* w, const gmh::Fp2& a) {
-    memcpy(w, a.c0.l, 56);
-    memcpy(w + 7, a.c1.l, 56);
-}
-void get_elem(gmh::Fp& a, const uint64_t* w) { memcpy(a.l, w, 56); }
-void get_elem(gmh::Fp2& a, const uint64_t* w) {
-    memcpy(a.c0.l, w, 56);
-    memcpy(a.c1.l, w + 7, 56);
-}
-
-// Writes an affine result as an ABI affine record or an ABI projective point.
-template <class HF>
-void write_affine_abi(void* out, bool finite, const HF& x, const HF& y) {
-    constexpr int E = sizeof(HF) / 8;
-    uint64_t buf[2 * E + 1];
-    memset(buf, 0, sizeof buf);
-    if (finite) {
-        put_elem(buf, x);
-        put_elem(buf + E, y);
-    } else {
-        buf[2 * E] = 1;
-    }
-    memcpy(out, buf, sizeof buf);
-}
-template <class HF>
-void write_proj_abi(void* out, bool finite, const HF& x, const HF& y) {
-    constexpr int E = sizeof(HF) / 8;
-    uint64_t buf[3 * E];
-    memset(buf, 0, sizeof buf);
-    if (finite) {
-        put_elem(buf, x);
-        put_elem(buf + E, y);
-        buf[2 * E + 0] = 0;
-        memcpy(buf + 2 * E, gmh::ONE.l, 56);
-    }
-    memcpy(out, buf, sizeof buf);
-}
-
-template <class O>
-struct HostOf;
-template <>
-struct HostOf<FpOps> {
-    using HF = gmh::Fp;
-    static constexpr int CURVE = gm::CURVE_G1;
-};
-template <>
-struct HostOf<Fp2Ops> {
-    using HF = gmh::Fp2;
-    static constexpr int CURVE = gm::CURVE_G2;
-};
-
-template <class HF>
-struct AffResult {
-    bool finite;
-    HF x, y;
-};
+void host_combine_many(std::vector<AffResult<HF>>& out, const uint32_t* raw, const gm::MsmDesc* desc, int count);
 
 // Runs the pipeline for a batch of MSMs over resident bases/scalars and
 // finishes each with a host (or device) window combine. Caller holds c->mu.

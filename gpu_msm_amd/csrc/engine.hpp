@@ -11,6 +11,9 @@ namespace gm {
 
 enum Curve { CURVE_G1 = 1, CURVE_G2 = 2 };
 
+// Most u64 words per scalar the digit kernels accept.
+constexpr int MAX_SCALAR_LIMBS = 8;
+
 // Reference ABI record sizes (reference include/curve.h:13-43).
 template <class O>
 struct AbiLayout;

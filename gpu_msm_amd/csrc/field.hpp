@@ -207,14 +207,12 @@ __device__ __forceinline__ bool fp_eq(const Fp& a, const Fp& b) {
     return fp_is_zero(d);
 }
 
-// a^(p-2) (Fermat inverse); 0 maps to 0.
-__device__ inline void fp_inv(Fp& r, const Fp& a) {
-    // p - 2 in 32-bit words, most significant first
-    constexpr uint32_t E[14] = {0x3cdee0fb, 0x28c5e535, 0x200fc349, 0x65aad640, 0x0095a4b7, 0x8a02fe32, 0x0f75a64b,
-                                0xbac71602, 0x824e6dc3, 0xe23acdee, 0x56ee4528, 0xc573b5cc, 0x311c0026, 0xaab0aaa9};
+// acc = a^E in the Montgomery domain; E is nw 32-bit words, most significant
+// first (left-to-right square and multiply).
+__device__ inline void fp_pow_words(Fp& r, const Fp& a, const uint32_t* E, int nw) {
     Fp acc;
     fp_set_one(acc);
-    for (int w = 0; w < 14; ++w) {
+    for (int w = 0; w < nw; ++w) {
         const uint32_t e = E[w];
         for (int b = 31; b >= 0; --b) {
             fp_sqr(acc, acc);
@@ -223,6 +221,14 @@ __device__ inline void fp_inv(Fp& r, const Fp& a) {
     }
     r = acc;
 }
+
+// p - 2 in 32-bit words, most significant first
+constexpr uint32_t PM2_WORDS[14] = {0x3cdee0fb, 0x28c5e535, 0x200fc349, 0x65aad640, 0x0095a4b7,
+                                    0x8a02fe32, 0x0f75a64b, 0xbac71602, 0x824e6dc3, 0xe23acdee,
+                                    0x56ee4528, 0xc573b5cc, 0x311c0026, 0xaab0aaa9};
+
+// a^(p-2) (Fermat inverse); 0 maps to 0.
+__device__ inline void fp_inv(Fp& r, const Fp& a) { fp_pow_words(r, a, PM2_WORDS, 14); }
 
 // ------------------------------------------------------------------ Fp2
 // Fp2 = Fp[i]/(i^2 + 1) (non-residue -1 as in reference include/fp2.h:7).

@@ -26,8 +26,6 @@
 
 namespace gm {
 
-constexpr int MAX_SCALAR_LIMBS = 8;
-
 template <class T>
 __device__ __forceinline__ void ld16(T& dst, const T* src) {
     static_assert(sizeof(T) % 16 == 0, "16-byte granularity");

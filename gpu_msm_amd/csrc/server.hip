@@ -26,6 +26,7 @@
 
 #include "../../include/gpu_msm_ext.h"
 #include "engine.hpp"
+#include "host_combine.hpp"
 #include "host_field.hpp"
 #include "resident.hpp"
 
@@ -76,52 +77,6 @@ struct Slot {
     size_t footprint = 0;
     size_t res_bytes = 0;
 };
-
-// ---- host side of the window combine (same arithmetic as capi.hip)
-
-void elem_from_dev(gmh::Fp& r, const uint32_t* v) {
-    gmh::Fp t;
-    gmh::from_limbs28(t, v);
-    gmh::reduce(r, t);
-}
-void elem_from_dev(gmh::Fp2& r, const uint32_t* v) {
-    elem_from_dev(r.c0, v);
-    elem_from_dev(r.c1, v + 16);
-}
-
-template <class HF>
-bool combine(HF& x, HF& y, const uint32_t* raw, int nw, int c) {
-    constexpr int W = sizeof(HF) == sizeof(gmh::Fp) ? 16 : 32;
-    auto load = [&](gmh::Xyzz<HF>& p, int w) {
-        const uint32_t* b = raw + (size_t)w * 4 * W;
-        elem_from_dev(p.X, b);
-        elem_from_dev(p.Y, b + W);
-        elem_from_dev(p.ZZ, b + 2 * W);
-        elem_from_dev(p.ZZZ, b + 3 * W);
-    };
-    gmh::Xyzz<HF> r, v;
-    load(r, nw - 1);
-    for (int w = nw - 2; w >= 0; --w) {
-        for (int k = 0; k < c; ++k) gmh::xdbl(r, r);
-        load(v, w);
-        gmh::xadd(r, r, v);
-    }
-    return gmh::to_affine(x, y, r);
-}
-
-template <class HF>
-void write_affine(uint8_t* out, bool finite, const HF& x, const HF& y) {
-    constexpr int E = sizeof(HF) / 8;
-    uint64_t buf[2 * E + 1];
-    memset(buf, 0, sizeof buf);
-    if (finite) {
-        memcpy(buf, &x, sizeof(HF));
-        memcpy(buf + E, &y, sizeof(HF));
-    } else {
-        buf[2 * E] = 1;
-    }
-    memcpy(out, buf, sizeof buf);
-}
 
 // ---- planner MLP (host inference; same network as csrc/mlp.hip)
 
@@ -383,12 +338,12 @@ void completer(gm_server* S) {
                     const gm::MsmDesc& d = sl.desc[j];
                     if (sl.curve == GM_CURVE_G1) {
                         gmh::Fp x, y;
-                        const bool f = combine<gmh::Fp>(x, y, raw + (size_t)d.win_base * 64, (int)d.nw, (int)d.c);
-                        write_affine<gmh::Fp>(sl.reqs[j]->result, f, x, y);
+                        const bool f = gmc::host_combine<gmh::Fp>(x, y, raw + (size_t)d.win_base * 64, (int)d.nw, (int)d.c);
+                        gmc::write_affine_abi<gmh::Fp>(sl.reqs[j]->result, f, x, y);
                     } else {
                         gmh::Fp2 x, y;
-                        const bool f = combine<gmh::Fp2>(x, y, raw + (size_t)d.win_base * 128, (int)d.nw, (int)d.c);
-                        write_affine<gmh::Fp2>(sl.reqs[j]->result, f, x, y);
+                        const bool f = gmc::host_combine<gmh::Fp2>(x, y, raw + (size_t)d.win_base * 128, (int)d.nw, (int)d.c);
+                        gmc::write_affine_abi<gmh::Fp2>(sl.reqs[j]->result, f, x, y);
                     }
                 }
             };

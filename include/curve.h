@@ -1,0 +1,133 @@
+/*
+ * C++ device-pointer API of the curve layer: a drop-in for the reference's
+ * include/curve.h:140-241 (same names, argument meaning and PointType
+ * instantiations; cudaStream_t becomes hipStream_t). Implemented in
+ * gpu_msm_amd/csrc/surface.hip on the MI355X kernels of this library.
+ *
+ * Every pointer is a DEVICE pointer. `_async` functions enqueue on `stream`
+ * and return; the others also synchronise `stream`. PointType is G1Point or
+ * G2Point: affine records in Montgomery form with field values below p.
+ * Results are canonical, so they equal the reference's bit for bit.
+ *
+ * Errors behave as the reference's PANIC / check_cuda_error
+ * (reference include/device.h:11-39): a null pointer, a negative size or a HIP
+ * error prints a message naming the function and calls abort().
+ */
+#ifndef GPU_MSM_CURVE_H
+#define GPU_MSM_CURVE_H
+
+#ifndef __cplusplus
+#error "curve.h is the C++ surface; C callers use rust_wrapper.h and gpu_msm_ext.h"
+#endif
+
+#include <hip/hip_runtime_api.h>
+#include <stddef.h>
+
+#include "gpu_msm_types.h"
+
+/* ------------------------------------------- single-point operations
+ * reference include/curve.h:150-206, src/curve.cu:2386-2525; formulas
+ * src/curve.cu:72-268 (affine add / double / negate / double-and-add). */
+
+/* d_result = d_p1 + d_p2; O + P copies P as stored (reference curve.cu:178-189). */
+template <typename PointType>
+void point_add_async(hipStream_t stream, uint32_t gpu_index, PointType* d_result, const PointType* d_p1,
+                     const PointType* d_p2);
+template <typename PointType>
+void point_add(hipStream_t stream, uint32_t gpu_index, PointType* d_result, const PointType* d_p1,
+               const PointType* d_p2);
+
+/* d_result = 2 d_p; O and points with y = 0 give O. */
+template <typename PointType>
+void point_double_async(hipStream_t stream, uint32_t gpu_index, PointType* d_result, const PointType* d_p);
+template <typename PointType>
+void point_double(hipStream_t stream, uint32_t gpu_index, PointType* d_result, const PointType* d_p);
+
+/* d_result = -d_p (x kept, y -> p - y). */
+template <typename PointType>
+void point_neg_async(hipStream_t stream, uint32_t gpu_index, PointType* d_result, const PointType* d_p);
+template <typename PointType>
+void point_neg(hipStream_t stream, uint32_t gpu_index, PointType* d_result, const PointType* d_p);
+
+/* d_result = scalar * d_point (one u64 scalar). */
+template <typename PointType>
+void point_scalar_mul_u64_async(hipStream_t stream, uint32_t gpu_index, PointType* d_result,
+                                const PointType* d_point, uint64_t scalar);
+template <typename PointType>
+void point_scalar_mul_u64(hipStream_t stream, uint32_t gpu_index, PointType* d_result, const PointType* d_point,
+                          uint64_t scalar);
+
+/* d_result = scalar * d_point, scalar = scalar_limbs little-endian u64 words at d_scalar (device). */
+template <typename PointType>
+void point_scalar_mul_async(hipStream_t stream, uint32_t gpu_index, PointType* d_result, const PointType* d_point,
+                            const uint64_t* d_scalar, int scalar_limbs);
+template <typename PointType>
+void point_scalar_mul(hipStream_t stream, uint32_t gpu_index, PointType* d_result, const PointType* d_point,
+                      const uint64_t* d_scalar, int scalar_limbs);
+
+/* d_result = O: x = y = 0, infinity = true. */
+template <typename PointType>
+void point_at_infinity_async(hipStream_t stream, uint32_t gpu_index, PointType* d_result);
+template <typename PointType>
+void point_at_infinity(hipStream_t stream, uint32_t gpu_index, PointType* d_result);
+
+/* Coordinates times R (to) or R^-1 (from) mod p; O gives O. */
+template <typename PointType>
+void point_to_montgomery_async(hipStream_t stream, uint32_t gpu_index, PointType* d_result, const PointType* d_point);
+template <typename PointType>
+void point_to_montgomery(hipStream_t stream, uint32_t gpu_index, PointType* d_result, const PointType* d_point);
+template <typename PointType>
+void point_from_montgomery_async(hipStream_t stream, uint32_t gpu_index, PointType* d_result,
+                                 const PointType* d_point);
+template <typename PointType>
+void point_from_montgomery(hipStream_t stream, uint32_t gpu_index, PointType* d_result, const PointType* d_point);
+
+/* In place over n records; records at infinity are left unchanged
+ * (reference curve.cu:2333-2352, 2506-2522). */
+template <typename PointType>
+void point_to_montgomery_batch_async(hipStream_t stream, uint32_t gpu_index, PointType* d_points, int n);
+template <typename PointType>
+void point_to_montgomery_batch(hipStream_t stream, uint32_t gpu_index, PointType* d_points, int n);
+
+/* ------------------------------------------------------------------ MSM
+ * reference include/curve.h:212-241, src/curve.cu:2538-2826.
+ *
+ * d_points: n affine bases in Montgomery form. d_result: projective
+ * (x, y, R mod p) or all-zero for O (u64 functions), or the canonical affine
+ * record (PointType functions). n = 0 gives O; n < 0 aborts.
+ *
+ * Scratch: the reference sizes d_scratch as (blocks + 1) * 16 points; this
+ * library's pipeline (sorted bucket lists, per-window buckets) needs the byte
+ * count returned by the *_scratch_bytes query for the same n (and limbs).
+ * A caller that cannot size it passes d_scratch = NULL: the call then takes
+ * and returns the memory stream-ordered (hipMallocAsync / hipFreeAsync).
+ *
+ * The _async forms finish entirely on the device (window combine included)
+ * and never block the host; the synchronous forms finish the window combine
+ * on the host, which is faster for a single MSM. Both give identical results.
+ * Calls on different streams may overlap when their scratch buffers differ. */
+size_t point_msm_u64_scratch_bytes_g1(int n);
+size_t point_msm_u64_scratch_bytes_g2(int n);
+template <typename PointType>
+size_t point_msm_scratch_bytes(int n, int scalar_limbs);
+
+void point_msm_u64_async_g1(hipStream_t stream, uint32_t gpu_index, G1ProjectivePoint* d_result,
+                            const G1Point* d_points, const uint64_t* d_scalars, G1ProjectivePoint* d_scratch, int n);
+void point_msm_u64_g1(hipStream_t stream, uint32_t gpu_index, G1ProjectivePoint* d_result, const G1Point* d_points,
+                      const uint64_t* d_scalars, G1ProjectivePoint* d_scratch, int n);
+void point_msm_u64_async_g2(hipStream_t stream, uint32_t gpu_index, G2ProjectivePoint* d_result,
+                            const G2Point* d_points, const uint64_t* d_scalars, G2ProjectivePoint* d_scratch, int n);
+void point_msm_u64_g2(hipStream_t stream, uint32_t gpu_index, G2ProjectivePoint* d_result, const G2Point* d_points,
+                      const uint64_t* d_scalars, G2ProjectivePoint* d_scratch, int n);
+
+/* Multi-limb scalars: scalar_limbs (1..8) little-endian u64 words per point,
+ * any value below 2^(64 scalar_limbs); affine result (reference
+ * src/curve.cu:2763-2826). */
+template <typename PointType>
+void point_msm_async(hipStream_t stream, uint32_t gpu_index, PointType* d_result, const PointType* d_points,
+                     const uint64_t* d_scalars, int scalar_limbs, PointType* d_scratch, int n);
+template <typename PointType>
+void point_msm(hipStream_t stream, uint32_t gpu_index, PointType* d_result, const PointType* d_points,
+               const uint64_t* d_scalars, int scalar_limbs, PointType* d_scratch, int n);
+
+#endif /* GPU_MSM_CURVE_H */
