@@ -57,9 +57,13 @@ struct MsmDesc {
     uint32_t win_base;   // first global window
     uint32_t chunk;      // entries per accumulation lane for this MSM
     uint32_t lane_base;  // first accumulation lane
-    uint32_t pad;
+    uint32_t tile_base;  // first window-sum tile (WSUM_TILE segments each)
 };
 static_assert(sizeof(MsmDesc) == 64, "MsmDesc layout");
+
+// Segments summed by one window-sum tile (64 threads x 4) before the
+// per-window total over tiles.
+constexpr uint32_t WSUM_TILE = 256;
 
 // What a caller asks for: c <= 0 and seg <= 0 pick the defaults.
 struct MsmSpec {
@@ -75,11 +79,11 @@ struct Plan {
     int L;          // u64 limbs per scalar (all MSMs of a batch)
     int bits;       // scalar bit bound (values must be < 2^bits)
     int count;      // MSMs in the batch
-    uint32_t n_total, ent_total, bkt_total, seg_total, win_total;
+    uint32_t n_total, ent_total, bkt_total, seg_total, win_total, tile_total;
     int lanes;      // accumulation lanes (upper bound: all digits non-zero)
     size_t cub_bytes;
     // workspace offsets (bytes)
-    size_t o_desc, o_cnt, o_off, o_rank, o_sorted, o_buckets, o_head, o_tail, o_heavy, o_segs, o_wsum, o_out, o_err,
+    size_t o_desc, o_cnt, o_off, o_rank, o_sorted, o_buckets, o_head, o_tail, o_heavy, o_segs, o_tiles, o_wsum, o_out, o_err,
         o_cub,
         total;
     MsmDesc* host_desc;  // owned by the caller of make_plan (count entries)

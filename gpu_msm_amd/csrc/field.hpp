@@ -246,17 +246,90 @@ __device__ __forceinline__ void fp2_neg(Fp2& r, const Fp2& a) {
     fp_neg(r.c0, a.c0);
     fp_neg(r.c1, a.c1);
 }
-// Karatsuba: t0 = a0 b0, t1 = a1 b1, t2 = (a0+a1)(b0+b1); c0 = t0 - t1, c1 = t2 - t0 - t1.
-__device__ __forceinline__ void fp2_mul(Fp2& r, const Fp2& a, const Fp2& b) {
-    Fp t0, t1, t2, sa, sb;
-    fp_add(sa, a.c0, a.c1);
-    fp_add(sb, b.c0, b.c1);
-    fp_mul(t0, a.c0, b.c0);
-    fp_mul(t1, a.c1, b.c1);
-    fp_mul(t2, sa, sb);
-    fp_sub(r.c0, t0, t1);
-    fp_sub(t2, t2, t0);
-    fp_sub(r.c1, t2, t1);
+// (a0 + a1 i)(b0 + b1 i) = (a0 b0 - a1 b1) + (a0 b1 + a1 b0) i with both
+// components reduced lazily: the two double-width products of each component
+// are accumulated into one column sum before a single Montgomery reduction,
+// so the product costs two reductions instead of three (Karatsuba over three
+// fully reduced Fp products) and no intermediate Fp add/sub.
+//   c0 columns are signed: a1 b1 enters through signed mads with -b1. With
+//   inputs in [0, 2p), |a0 b0 - a1 b1| < 4p^2, so the reduced value lies in
+//   (-4p^2/R, 4p^2/R + p) within (-p, 2p); a negative result gets 2p added.
+//   c1 = a0 b1 + a1 b0 < 8p^2 reduces to below 8p^2/R + p < 3p; one
+//   conditional subtraction of 2p returns it to [0, 2p).
+// Column bounds: limbs < 2^28, so each product < 2^56; 32 products plus 16
+// reduction terms stay below 2^62 in magnitude.
+__host__ __device__ __forceinline__ void fp2_mul(Fp2& r, const Fp2& a, const Fp2& b) {
+    int32_t nb1[NL];
+#pragma unroll
+    for (int i = 0; i < NL; ++i) nb1[i] = -(int32_t)b.c1.v[i];
+    uint32_t m0[NL], m1[NL];
+    int64_t acc0 = 0;
+    uint64_t acc1 = 0;
+#pragma unroll
+    for (int k = 0; k < NL; ++k) {
+#pragma unroll
+        for (int i = 0; i <= k; ++i) {
+            acc0 += (int64_t)(int32_t)a.c0.v[i] * (int32_t)b.c0.v[k - i];
+            acc0 += (int64_t)(int32_t)a.c1.v[i] * nb1[k - i];
+            acc1 += (uint64_t)a.c0.v[i] * b.c1.v[k - i];
+            acc1 += (uint64_t)a.c1.v[i] * b.c0.v[k - i];
+        }
+#pragma unroll
+        for (int i = 0; i < k; ++i) {
+            acc0 += (int64_t)((uint64_t)m0[i] * PL[k - i]);
+            acc1 += (uint64_t)m1[i] * PL[k - i];
+        }
+        m0[k] = ((uint32_t)acc0 * PINV28) & MASK28;
+        m1[k] = ((uint32_t)acc1 * PINV28) & MASK28;
+        acc0 += (int64_t)((uint64_t)m0[k] * PL[0]);
+        acc1 += (uint64_t)m1[k] * PL[0];
+        acc0 >>= 28;  // arithmetic: the low 28 bits are zero
+        acc1 >>= 28;
+    }
+    uint32_t r0[NL], r1[NL];
+#pragma unroll
+    for (int k = NL; k < 2 * NL - 1; ++k) {
+#pragma unroll
+        for (int i = k - NL + 1; i < NL; ++i) {
+            acc0 += (int64_t)(int32_t)a.c0.v[i] * (int32_t)b.c0.v[k - i];
+            acc0 += (int64_t)(int32_t)a.c1.v[i] * nb1[k - i];
+            acc1 += (uint64_t)a.c0.v[i] * b.c1.v[k - i];
+            acc1 += (uint64_t)a.c1.v[i] * b.c0.v[k - i];
+            acc0 += (int64_t)((uint64_t)m0[i] * PL[k - i]);
+            acc1 += (uint64_t)m1[i] * PL[k - i];
+        }
+        r0[k - NL] = (uint32_t)acc0 & MASK28;
+        r1[k - NL] = (uint32_t)acc1 & MASK28;
+        acc0 >>= 28;
+        acc1 >>= 28;
+    }
+    // c0: top limb carries the sign; add 2p when negative.
+    {
+        const uint32_t msk = acc0 < 0 ? 0xFFFFFFFFu : 0u;
+        int32_t c = 0;
+#pragma unroll
+        for (int i = 0; i < NL - 1; ++i) {
+            const int32_t t = (int32_t)r0[i] + (int32_t)(P2L[i] & msk) + c;
+            r.c0.v[i] = (uint32_t)t & MASK28;
+            c = t >> 28;
+        }
+        r.c0.v[NL - 1] = (uint32_t)((int32_t)acc0 + (int32_t)(P2L[NL - 1] & msk) + c);
+    }
+    // c1: subtract 2p unless that borrows.
+    {
+        r1[NL - 1] = (uint32_t)acc1;
+        uint32_t d[NL];
+        int32_t br = 0;
+#pragma unroll
+        for (int i = 0; i < NL; ++i) {
+            const int32_t t = (int32_t)r1[i] - (int32_t)P2L[i] + br;
+            d[i] = (uint32_t)t & MASK28;
+            br = t >> 28;
+        }
+        const bool keep = br < 0;
+#pragma unroll
+        for (int i = 0; i < NL; ++i) r.c1.v[i] = keep ? r1[i] : d[i];
+    }
 }
 // (a0 + a1 i)^2 = (a0 + a1)(a0 - a1) + 2 a0 a1 i
 __device__ __forceinline__ void fp2_sqr(Fp2& r, const Fp2& a) {

@@ -16,7 +16,8 @@
 //                  them into an XYZZ accumulator
 //   k_bucket_reduce one lane per segment of buckets: running sums give
 //                  sum_b (b+1) B_b for the segment as T + lo * S
-//   k_window_sum   one workgroup per window: tree sum of the segment results
+//   k_window_partial / k_window_sum  tiles of segments, then each window's
+//                  total over its tiles (two short tree stages)
 //   combine        Horner over windows, most significant first (reference
 //                  src/curve.cu:2060-2179 combines on the CPU after every
 //                  window; here either on the host once, or on the device)
@@ -346,22 +347,62 @@ __global__ void __launch_bounds__(64) k_bucket_reduce(const MsmDesc* __restrict_
     st16(&segs[g], T);
 }
 
-// One workgroup per (MSM, window): tree sum of its segment results.
+// Window sums in two short stages, so the serial depth stays small and the
+// whole GPU takes part (one workgroup per window would leave most CUs idle):
+//   k_window_partial  one 64-thread workgroup per tile of WSUM_TILE segments
+//                     of one window: 4 strided adds per thread, then a tree
+//                     in LDS;
+//   k_window_sum      one workgroup per (MSM, window): tree over its tiles.
+template <class O>
+__global__ void __launch_bounds__(64) k_window_partial(const MsmDesc* __restrict__ desc, int count,
+                                                       const Xyzz<O>* __restrict__ segs,
+                                                       Xyzz<O>* __restrict__ tiles) {
+    constexpr int T = 64;
+    __shared__ Xyzz<O> sh[T];
+    const uint32_t gt = blockIdx.x;
+    const int tid = threadIdx.x;
+    const MsmDesc& m = desc[GM_FIND(desc, count, tile_base, gt)];
+    const uint32_t ntile = (m.nseg + WSUM_TILE - 1) / WSUM_TILE;
+    const uint32_t local = gt - m.tile_base;
+    const uint32_t w = local / ntile, k = local - w * ntile;
+    const Xyzz<O>* sw = segs + m.seg_base + (size_t)w * m.nseg;
+    const uint32_t t1 = min((k + 1) * WSUM_TILE, m.nseg);
+    Xyzz<O> acc;
+    xyzz_set_inf(acc);
+    for (uint32_t t = k * WSUM_TILE + tid; t < t1; t += T) {
+        Xyzz<O> v;
+        ld16(v, &sw[t]);
+        xyzz_add<O>(acc, acc, v);
+    }
+    sh[tid] = acc;
+    __syncthreads();
+    for (int s = T / 2; s > 0; s >>= 1) {
+        if (tid < s) {
+            Xyzz<O> a = sh[tid], b = sh[tid + s];
+            xyzz_add<O>(a, a, b);
+            sh[tid] = a;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) st16(&tiles[gt], sh[0]);
+}
+
 template <class O, int THREADS>
 __global__ void __launch_bounds__(THREADS) k_window_sum(const MsmDesc* __restrict__ desc, int count,
-                                                        const Xyzz<O>* __restrict__ segs,
+                                                        const Xyzz<O>* __restrict__ tiles,
                                                         Xyzz<O>* __restrict__ wsum) {
     __shared__ Xyzz<O> sh[THREADS];
     const uint32_t gw = blockIdx.x;
     const int tid = threadIdx.x;
     const MsmDesc& m = desc[GM_FIND(desc, count, win_base, gw)];
     const uint32_t w = gw - m.win_base;
-    const Xyzz<O>* sw = segs + m.seg_base + (size_t)w * m.nseg;
+    const uint32_t ntile = (m.nseg + WSUM_TILE - 1) / WSUM_TILE;
+    const Xyzz<O>* tw = tiles + m.tile_base + (size_t)w * ntile;
     Xyzz<O> acc;
     xyzz_set_inf(acc);
-    for (uint32_t t = tid; t < m.nseg; t += THREADS) {
+    for (uint32_t t = tid; t < ntile; t += THREADS) {
         Xyzz<O> v;
-        ld16(v, &sw[t]);
+        ld16(v, &tw[t]);
         xyzz_add<O>(acc, acc, v);
     }
     sh[tid] = acc;

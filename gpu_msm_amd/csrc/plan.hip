@@ -59,7 +59,7 @@ bool make_plan(Plan& p, MsmDesc* d, int curve, int L, int bits, const MsmSpec* s
     p.bits = bits;
     p.count = count;
     p.host_desc = d;
-    uint64_t pt = 0, rank = 0, bkt = 0, seg = 0, win = 0;
+    uint64_t pt = 0, rank = 0, bkt = 0, seg = 0, win = 0, tile = 0;
     for (int j = 0; j < count; ++j) {
         const MsmSpec& s = specs[j];
         if (s.n < 0 || s.base_off < 0 || s.scal_off < 0) return false;
@@ -85,11 +85,13 @@ bool make_plan(Plan& p, MsmDesc* d, int curve, int L, int bits, const MsmSpec* s
         m.rank_base = (uint32_t)rank;
         m.seg_base = (uint32_t)seg;
         m.win_base = (uint32_t)win;
+        m.tile_base = (uint32_t)tile;
         pt += (uint64_t)s.n;
         rank += (uint64_t)nw * s.n;
         bkt += (uint64_t)nw * B;
         seg += (uint64_t)nw * (B / sg);
         win += (uint64_t)nw;
+        tile += (uint64_t)nw * ((m.nseg + WSUM_TILE - 1) / WSUM_TILE);
         if (rank >= 0xFFFFFFFFull || bkt >= 0x7FFFFFFFull) return false;
     }
     p.n_total = (uint32_t)pt;
@@ -97,12 +99,15 @@ bool make_plan(Plan& p, MsmDesc* d, int curve, int L, int bits, const MsmSpec* s
     p.bkt_total = (uint32_t)bkt;
     p.seg_total = (uint32_t)seg;
     p.win_total = (uint32_t)win;
+    p.tile_total = (uint32_t)tile;
     // Entries per accumulation lane, per MSM (chunks never straddle two
     // MSMs). Parallelism asks for ~64k lanes over the whole batch (chunk 32
-    // for big runs, down to 2 for small ones, so the serial depth of one mixed
-    // add per entry stays short); the fix-up asks that a typical bucket of the
-    // MSM spans at most ~4 chunks, so chunk >= the MSM's average bucket / 4.
-    int k_par = 32;
+    // for big G1 runs, 64 for G2, whose fix-up adds cost three times more and
+    // whose longer mixed adds hide the gathers with fewer lanes; down to 2 for
+    // small runs, so the serial depth of one mixed add per entry stays
+    // short); the fix-up asks that a typical bucket of the MSM spans at most
+    // ~4 chunks, so chunk >= the MSM's average bucket / 4.
+    int k_par = curve == CURVE_G2 ? 64 : 32;
     while (k_par > 2 && rank < (uint64_t)k_par * 65536) k_par >>= 1;
     uint64_t lanes = 0;
     for (int j = 0; j < count; ++j) {
@@ -142,6 +147,7 @@ bool make_plan(Plan& p, MsmDesc* d, int curve, int L, int bits, const MsmSpec* s
     p.o_tail = take((size_t)p.lanes * xb);
     p.o_heavy = take((bkt + 1) * 4);
     p.o_segs = take(seg * xb);
+    p.o_tiles = take(tile * xb);
     p.o_wsum = take(win * xb);
     p.o_out = take((size_t)count * aff_bytes(curve));
     p.o_err = take(8);  // error flag, heavy-bucket count

@@ -15,10 +15,12 @@ hipError_t launch_reduce(hipStream_t s, const Plan& p, uint8_t* ws) {
 
 template <class O>
 hipError_t launch_window_sum(hipStream_t s, const Plan& p, uint8_t* ws) {
-    constexpr int WT = sizeof(Xyzz<O>) > 256 ? 128 : 256;
     if (p.win_total == 0) return hipSuccess;
-    k_window_sum<O, WT><<<p.win_total, WT, 0, s>>>(reinterpret_cast<const MsmDesc*>(ws + p.o_desc), p.count,
-                                                   reinterpret_cast<const Xyzz<O>*>(ws + p.o_segs),
+    const MsmDesc* desc = reinterpret_cast<const MsmDesc*>(ws + p.o_desc);
+    k_window_partial<O><<<p.tile_total, 64, 0, s>>>(desc, p.count, reinterpret_cast<const Xyzz<O>*>(ws + p.o_segs),
+                                                    reinterpret_cast<Xyzz<O>*>(ws + p.o_tiles));
+    GM_TRY(hipGetLastError());
+    k_window_sum<O, 16><<<p.win_total, 16, 0, s>>>(desc, p.count, reinterpret_cast<const Xyzz<O>*>(ws + p.o_tiles),
                                                    reinterpret_cast<Xyzz<O>*>(ws + p.o_wsum));
     return hipGetLastError();
 }
