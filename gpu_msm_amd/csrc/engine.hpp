@@ -40,7 +40,7 @@ struct Prof {
     bool on;
 };
 
-// One MSM of a batch, as the kernels see it (64 B, device resident).
+// One MSM of a batch, as the kernels see it (80 B, device resident).
 struct MsmDesc {
     uint32_t n;          // points
     uint32_t base_off;   // first base in the resident pool
@@ -58,12 +58,23 @@ struct MsmDesc {
     uint32_t chunk;      // entries per accumulation lane for this MSM
     uint32_t lane_base;  // first accumulation lane
     uint32_t tile_base;  // first window-sum tile (WSUM_TILE segments each)
+    uint32_t stile_base; // first sort tile (stile_pts points of one window)
+    uint32_t stile_pts;  // points per sort tile
+    uint32_t tb_base;    // first per-tile bucket base (B words per sort tile)
+    uint32_t pad;
 };
-static_assert(sizeof(MsmDesc) == 64, "MsmDesc layout");
+static_assert(sizeof(MsmDesc) == 80, "MsmDesc layout");
 
 // Segments summed by one window-sum tile (64 threads x 4) before the
 // per-window total over tiles.
 constexpr uint32_t WSUM_TILE = 256;
+
+// Counting sort with per-tile bucket counters in LDS: used while every MSM of
+// a batch has c <= SORT_LDS_MAX_C (2^(c-1) counters of 4 bytes per
+// workgroup); larger windows fall back to one global atomic per entry.
+constexpr int SORT_LDS_MAX_C = 16;
+// Points of one window counted by one sort workgroup.
+constexpr uint32_t SORT_TILE_PTS = 1u << 17;
 
 // What a caller asks for: c <= 0 and seg <= 0 pick the defaults.
 struct MsmSpec {
@@ -80,10 +91,12 @@ struct Plan {
     int bits;       // scalar bit bound (values must be < 2^bits)
     int count;      // MSMs in the batch
     uint32_t n_total, ent_total, bkt_total, seg_total, win_total, tile_total;
+    uint32_t stile_total;  // sort tiles (0: the global-atomic fallback sort)
+    uint32_t lds_B;        // largest bucket count of the batch (LDS counters per sort tile)
     int lanes;      // accumulation lanes (upper bound: all digits non-zero)
     size_t cub_bytes;
     // workspace offsets (bytes)
-    size_t o_desc, o_cnt, o_off, o_rank, o_sorted, o_buckets, o_head, o_tail, o_heavy, o_segs, o_tiles, o_wsum, o_out, o_err,
+    size_t o_desc, o_cnt, o_off, o_rank, o_dig, o_tbase, o_sorted, o_buckets, o_head, o_tail, o_heavy, o_segs, o_tiles, o_wsum, o_out, o_err,
         o_cub,
         total;
     MsmDesc* host_desc;  // owned by the caller of make_plan (count entries)

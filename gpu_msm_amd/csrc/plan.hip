@@ -94,6 +94,28 @@ bool make_plan(Plan& p, MsmDesc* d, int curve, int L, int bits, const MsmSpec* s
         tile += (uint64_t)nw * ((m.nseg + WSUM_TILE - 1) / WSUM_TILE);
         if (rank >= 0xFFFFFFFFull || bkt >= 0x7FFFFFFFull) return false;
     }
+    // Sort tiles: every MSM of the batch must fit its counters in LDS.
+    bool lds_sort = true;
+    uint32_t maxB = 0;
+    for (int j = 0; j < count; ++j) {
+        lds_sort = lds_sort && (int)d[j].c <= SORT_LDS_MAX_C;
+        maxB = std::max(maxB, d[j].B);
+    }
+    uint64_t stile = 0, tb = 0;
+    if (lds_sort) {
+        for (int j = 0; j < count; ++j) {
+            MsmDesc& m = d[j];
+            m.stile_pts = std::max<uint32_t>(1, std::min<uint32_t>(m.n, SORT_TILE_PTS));
+            const uint64_t per_w = m.n ? ((uint64_t)m.n + m.stile_pts - 1) / m.stile_pts : 0;
+            m.stile_base = (uint32_t)stile;
+            m.tb_base = (uint32_t)tb;
+            stile += per_w * m.nw;
+            tb += per_w * m.nw * m.B;
+        }
+        if (tb >= 0xFFFFFFFFull) return false;
+    }
+    p.stile_total = (uint32_t)stile;
+    p.lds_B = lds_sort ? maxB : 0;
     p.n_total = (uint32_t)pt;
     p.ent_total = (uint32_t)rank;
     p.bkt_total = (uint32_t)bkt;
@@ -140,7 +162,10 @@ bool make_plan(Plan& p, MsmDesc* d, int curve, int L, int bits, const MsmSpec* s
     p.o_desc = take((size_t)count * sizeof(MsmDesc));
     p.o_cnt = take((bkt + 1) * 4);
     p.o_off = take((bkt + 1) * 4);
-    p.o_rank = take(rank * 4 + 4);
+    // entry ranks (global-atomic sort) or digit codes + per-tile bucket bases
+    p.o_rank = take(p.lds_B ? 0 : rank * 4 + 4);
+    p.o_dig = take(p.lds_B ? rank * 2 + 4 : 0);
+    p.o_tbase = take(p.lds_B ? tb * 4 + 4 : 0);
     p.o_sorted = take(rank * 8 + 8);
     p.o_buckets = take(bkt * xb);
     p.o_head = take((size_t)p.lanes * xb);

@@ -29,19 +29,51 @@ hipError_t launch_sort(hipStream_t s, const Plan& p, const uint64_t* d_scalars, 
     // can be captured in a hipGraph and replayed.
     GM_TRY(launch_fill_u32(s, cnt, nbk + 1, 0));
     GM_TRY(launch_fill_u32(s, err, 2, 0));  // error flag and heavy-bucket count
+    const bool tiled = p.lds_B > 0;
+    const size_t lds = (size_t)p.lds_B * 4;
+    uint16_t* dig = reinterpret_cast<uint16_t*>(ws + p.o_dig);
+    uint32_t* tbase = reinterpret_cast<uint32_t*>(ws + p.o_tbase);
+    if (tiled && lds > 65536) {
+        // above the default dynamic LDS limit; once per process per kernel
+        static const hipError_t attr = [] {
+            hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_tile_count),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, 4 << (SORT_LDS_MAX_C - 1));
+            if (e == hipSuccess)
+                e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_tile_place),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, 4 << (SORT_LDS_MAX_C - 1));
+            return e;
+        }();
+        GM_TRY(attr);
+    }
     if (p.n_total > 0) {
-        k_digits_hist<<<blocks_for(p.n_total, 256), 256, 0, s>>>(desc, p.count, p.n_total, d_scalars, p.L, cnt, rank,
-                                                                 err);
-        GM_TRY(hipGetLastError());
+        if (tiled) {
+            k_digits<<<blocks_for(p.n_total, 256), 256, 0, s>>>(desc, p.count, p.n_total, d_scalars, p.L, dig, err);
+            GM_TRY(hipGetLastError());
+            if (p.stile_total > 0) {
+                k_tile_count<<<p.stile_total, 256, lds, s>>>(desc, p.count, dig, cnt, tbase);
+                GM_TRY(hipGetLastError());
+            }
+        } else {
+            k_digits_hist<<<blocks_for(p.n_total, 256), 256, 0, s>>>(desc, p.count, p.n_total, d_scalars, p.L, cnt,
+                                                                     rank, err);
+            GM_TRY(hipGetLastError());
+        }
     }
     GM_TRY(mark(ST_HIST));
     size_t cub_bytes = p.cub_bytes;
     GM_TRY(hipcub::DeviceScan::ExclusiveSum(ws + p.o_cub, cub_bytes, cnt, off, (int)(nbk + 1), s));
     GM_TRY(mark(ST_SCAN));
     if (p.n_total > 0) {
-        k_scatter<<<blocks_for(p.n_total, 256), 256, 0, s>>>(desc, p.count, p.n_total, d_scalars, p.L, off, rank,
-                                                             sorted);
-        GM_TRY(hipGetLastError());
+        if (tiled) {
+            if (p.stile_total > 0) {
+                k_tile_place<<<p.stile_total, 256, lds, s>>>(desc, p.count, dig, off, tbase, sorted);
+                GM_TRY(hipGetLastError());
+            }
+        } else {
+            k_scatter<<<blocks_for(p.n_total, 256), 256, 0, s>>>(desc, p.count, p.n_total, d_scalars, p.L, off,
+                                                                 rank, sorted);
+            GM_TRY(hipGetLastError());
+        }
     }
     GM_TRY(mark(ST_SCATTER));
     return hipSuccess;

@@ -109,6 +109,103 @@ __global__ void __launch_bounds__(256) k_scatter(const MsmDesc* __restrict__ des
     }
 }
 
+// ------------------------------------------- tiled counting sort (c <= 16)
+//
+// k_digits      signed digits of every (window, point), coded in 16 bits:
+//               (|d| - 1) | sign << 15, DIG_ZERO for d = 0 (|d| <= 2^15 and
+//               |d| = 2^15 only occurs with a positive sign, so the code of
+//               zero is free);
+// k_tile_count  one workgroup per (MSM, window, tile of stile_pts points):
+//               bucket counts in LDS, then one global atomic per non-empty
+//               bucket of the tile, whose return value is the tile's base
+//               inside the bucket;
+// scan          exclusive prefix sum of the global counts;
+// k_tile_place  the same tiles again: LDS counters start at
+//               off[bucket] + tile base, and an LDS atomic per entry returns
+//               its final position.
+// Global atomics drop from one per entry to one per (tile, bucket).
+constexpr uint16_t DIG_ZERO = 0xFFFF;
+
+__global__ void __launch_bounds__(256) k_digits(const MsmDesc* __restrict__ desc, int count, uint32_t n_total,
+                                                const uint64_t* __restrict__ scalars, int L,
+                                                uint16_t* __restrict__ dig, uint32_t* __restrict__ err) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n_total) return;
+    const MsmDesc& m = desc[GM_FIND(desc, count, pt_base, p)];
+    const uint32_t i = p - m.pt_base;
+    DigitIter it;
+    it.load(scalars + (size_t)(m.scal_off + i) * L, L, (int)m.c);
+    for (uint32_t w = 0; w < m.nw; ++w) {
+        const int d = it.next();
+        const uint16_t code =
+            d == 0 ? DIG_ZERO : (uint16_t)(((uint32_t)(d < 0 ? -d : d) - 1) | (d < 0 ? 0x8000u : 0u));
+        dig[(size_t)m.rank_base + (size_t)w * m.n + i] = code;
+    }
+    if (!it.fits()) atomicOr(err, 1u);
+}
+
+// (MSM, window, tile) of sort workgroup gt, and the tile's point range.
+struct SortTile {
+    uint32_t w, local, i0, i1;
+};
+__device__ __forceinline__ SortTile sort_tile(const MsmDesc& m, uint32_t gt) {
+    const uint32_t per_w = (m.n + m.stile_pts - 1) / m.stile_pts;
+    SortTile t;
+    t.local = gt - m.stile_base;
+    t.w = t.local / per_w;
+    const uint32_t k = t.local - t.w * per_w;
+    t.i0 = k * m.stile_pts;
+    t.i1 = min(t.i0 + m.stile_pts, m.n);
+    return t;
+}
+
+__global__ void __launch_bounds__(256) k_tile_count(const MsmDesc* __restrict__ desc, int count,
+                                                    const uint16_t* __restrict__ dig, uint32_t* __restrict__ cnt,
+                                                    uint32_t* __restrict__ tbase) {
+    extern __shared__ uint32_t lc[];
+    const uint32_t gt = blockIdx.x;
+    const MsmDesc& m = desc[GM_FIND(desc, count, stile_base, gt)];
+    const SortTile t = sort_tile(m, gt);
+    for (uint32_t b = threadIdx.x; b < m.B; b += blockDim.x) lc[b] = 0;
+    __syncthreads();
+    const uint16_t* dw = dig + (size_t)m.rank_base + (size_t)t.w * m.n;
+    for (uint32_t i = t.i0 + threadIdx.x; i < t.i1; i += blockDim.x) {
+        const uint32_t code = dw[i];
+        if (code != DIG_ZERO) atomicAdd(&lc[code & 0x7FFFu], 1u);
+    }
+    __syncthreads();
+    uint32_t* tb = tbase + m.tb_base + (size_t)t.local * m.B;
+    uint32_t* gc = cnt + m.bkt_base + (size_t)t.w * m.B;
+    for (uint32_t b = threadIdx.x; b < m.B; b += blockDim.x) {
+        const uint32_t c = lc[b];
+        tb[b] = c ? atomicAdd(&gc[b], c) : 0u;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_tile_place(const MsmDesc* __restrict__ desc, int count,
+                                                    const uint16_t* __restrict__ dig,
+                                                    const uint32_t* __restrict__ off,
+                                                    const uint32_t* __restrict__ tbase,
+                                                    uint64_t* __restrict__ sorted) {
+    extern __shared__ uint32_t pos[];
+    const uint32_t gt = blockIdx.x;
+    const MsmDesc& m = desc[GM_FIND(desc, count, stile_base, gt)];
+    const SortTile t = sort_tile(m, gt);
+    const uint32_t g0 = m.bkt_base + t.w * m.B;
+    const uint32_t* tb = tbase + m.tb_base + (size_t)t.local * m.B;
+    for (uint32_t b = threadIdx.x; b < m.B; b += blockDim.x) pos[b] = off[g0 + b] + tb[b];
+    __syncthreads();
+    const uint16_t* dw = dig + (size_t)m.rank_base + (size_t)t.w * m.n;
+    for (uint32_t i = t.i0 + threadIdx.x; i < t.i1; i += blockDim.x) {
+        const uint32_t code = dw[i];
+        if (code == DIG_ZERO) continue;
+        const uint32_t b = code & 0x7FFFu;
+        const uint32_t at = atomicAdd(&pos[b], 1u);
+        // (global bucket << 32) | base index in the pool | sign bit
+        sorted[at] = ((uint64_t)(g0 + b) << 32) | (m.base_off + i) | ((code & 0x8000u) << 16);
+    }
+}
+
 __global__ void __launch_bounds__(256) k_fill_u32(uint32_t* __restrict__ p, size_t n, uint32_t v) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
