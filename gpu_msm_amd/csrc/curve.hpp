@@ -8,7 +8,8 @@
 // formulas (reference src/curve.cu:444-778).
 //
 // Affine inputs use an in-band infinity marker: limb 15 of x set to
-// 0xFFFFFFFF, a value no field element in [0, 2p) can take.
+// 0xFFFFFFFF, a value no field element in [0, 2p) can take (for G2 in both
+// components of x, so either lane of a pair sees it).
 #pragma once
 #include "field.hpp"
 
@@ -35,6 +36,11 @@ __device__ __forceinline__ bool aff_is_inf<FpOps>(const Aff<FpOps>& p) {
 template <>
 __device__ __forceinline__ bool aff_is_inf<Fp2Ops>(const Aff<Fp2Ops>& p) {
     return p.x.c0.v[NL - 1] == INF_MARK;
+}
+// Each lane of a pair sees its own component; the marker is in both.
+template <>
+__device__ __forceinline__ bool aff_is_inf<Fp2LaneOps>(const Aff<Fp2LaneOps>& p) {
+    return p.x.v[NL - 1] == INF_MARK;
 }
 
 template <class O>

@@ -400,4 +400,137 @@ struct Fp2Ops {
     }
 };
 
+// ------------------------------------------------- Fp2 over a lane pair
+// G2 alternative to Fp2Ops for the bucket kernels: lanes 2k and 2k+1 of a
+// wave carry components c0 and c1 of the same Fp2 values, so each lane holds
+// half the state of a point (a G2 XYZZ point is 64 registers per lane instead
+// of 128), which keeps the mixed addition below 256 registers and two waves
+// per SIMD in flight. A product swaps the partner's operand components with a
+// DPP quad permutation and each lane reduces its own component:
+//   part 0: a0 b0 - a1 b1,   part 1: a1 b0 + a0 b1,
+// both as x y + u w with signed w (w = -b1 for part 0, +b1 for part 1), one
+// fused column sum and one Montgomery reduction per lane; the result is
+// brought into [0, 2p) by the two corrections of fp2_mul (part 0 lies in
+// (-p, 2p), part 1 in [0, 3p)). Every branch that depends on a value
+// combines both lanes first, so control flow is uniform over each pair and
+// the DPP partner is always active.
+
+__device__ __forceinline__ uint32_t lane_part() { return threadIdx.x & 1u; }
+
+// Value of v in the partner lane (quad_perm [1, 0, 3, 2]).
+__device__ __forceinline__ uint32_t pair_swap(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false);
+}
+__device__ __forceinline__ void fp_pair_swap(Fp& r, const Fp& a) {
+#pragma unroll
+    for (int i = 0; i < NL; ++i) r.v[i] = pair_swap(a.v[i]);
+}
+
+// Own component of (a0 + a1 i)(b0 + b1 i); ao / bo are the partner's parts.
+__device__ __forceinline__ void fp2_lane_mul(Fp& r, const Fp& a, const Fp& ao, const Fp& b, const Fp& bo) {
+    const bool q = lane_part() != 0;
+    uint32_t y[NL];
+    int32_t w[NL];
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+        y[i] = q ? bo.v[i] : b.v[i];
+        const int32_t v = (int32_t)(q ? b.v[i] : bo.v[i]);
+        w[i] = q ? v : -v;
+    }
+    uint32_t m[NL];
+    int64_t acc = 0;
+#pragma unroll
+    for (int k = 0; k < NL; ++k) {
+#pragma unroll
+        for (int i = 0; i <= k; ++i) {
+            acc += (int64_t)(int32_t)a.v[i] * (int32_t)y[k - i];
+            acc += (int64_t)(int32_t)ao.v[i] * w[k - i];
+        }
+#pragma unroll
+        for (int i = 0; i < k; ++i) acc += (int64_t)((uint64_t)m[i] * PL[k - i]);
+        m[k] = ((uint32_t)acc * PINV28) & MASK28;
+        acc += (int64_t)((uint64_t)m[k] * PL[0]);
+        acc >>= 28;
+    }
+    uint32_t t[NL];
+#pragma unroll
+    for (int k = NL; k < 2 * NL - 1; ++k) {
+#pragma unroll
+        for (int i = k - NL + 1; i < NL; ++i) {
+            acc += (int64_t)(int32_t)a.v[i] * (int32_t)y[k - i];
+            acc += (int64_t)(int32_t)ao.v[i] * w[k - i];
+            acc += (int64_t)((uint64_t)m[i] * PL[k - i]);
+        }
+        t[k - NL] = (uint32_t)acc & MASK28;
+        acc >>= 28;
+    }
+    // add 2p when negative (part 0), then subtract 2p unless that borrows
+    const uint32_t msk = acc < 0 ? 0xFFFFFFFFu : 0u;
+    int32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < NL - 1; ++i) {
+        const int32_t v = (int32_t)t[i] + (int32_t)(P2L[i] & msk) + c;
+        t[i] = (uint32_t)v & MASK28;
+        c = v >> 28;
+    }
+    t[NL - 1] = (uint32_t)((int32_t)acc + (int32_t)(P2L[NL - 1] & msk) + c);
+    uint32_t d[NL];
+    int32_t br = 0;
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+        const int32_t v = (int32_t)t[i] - (int32_t)P2L[i] + br;
+        d[i] = (uint32_t)v & MASK28;
+        br = v >> 28;
+    }
+    const bool keep = br < 0;
+#pragma unroll
+    for (int i = 0; i < NL; ++i) r.v[i] = keep ? t[i] : d[i];
+}
+
+struct Fp2LaneOps {
+    using F = Fp;
+    static constexpr int WORDS = NL;
+    __device__ static __forceinline__ void add(F& r, const F& a, const F& b) { fp_add(r, a, b); }
+    __device__ static __forceinline__ void sub(F& r, const F& a, const F& b) { fp_sub(r, a, b); }
+    __device__ static __forceinline__ void dbl(F& r, const F& a) { fp_dbl(r, a); }
+    __device__ static __forceinline__ void neg(F& r, const F& a) { fp_neg(r, a); }
+    __device__ static __forceinline__ void mul(F& r, const F& a, const F& b) {
+        Fp ao, bo;
+        fp_pair_swap(ao, a);
+        fp_pair_swap(bo, b);
+        fp2_lane_mul(r, a, ao, b, bo);
+    }
+    __device__ static __forceinline__ void sqr(F& r, const F& a) {
+        Fp ao;
+        fp_pair_swap(ao, a);
+        fp2_lane_mul(r, a, ao, a, ao);
+    }
+    __device__ static __forceinline__ bool is_zero(const F& a) {
+        const uint32_t z = fp_is_zero(a) ? 1u : 0u;
+        return (z & pair_swap(z)) != 0;
+    }
+    __device__ static __forceinline__ bool is_raw_zero(const F& a) {
+        const uint32_t z = fp_is_raw_zero(a) ? 1u : 0u;
+        return (z & pair_swap(z)) != 0;
+    }
+    __device__ static __forceinline__ void canon(F& r, const F& a) { fp_canon(r, a); }
+    // (a0 - a1 i) / (a0^2 + a1^2): both lanes form the norm and invert it
+    __device__ static __forceinline__ void inv(F& r, const F& a) {
+        Fp s, so, n;
+        fp_sqr(s, a);
+        fp_pair_swap(so, s);
+        fp_add(n, s, so);
+        fp_inv(n, n);
+        fp_mul(r, a, n);
+        if (lane_part()) fp_neg(r, r);
+    }
+    __device__ static __forceinline__ void zero(F& r) { fp_set_zero(r); }
+    __device__ static __forceinline__ void one(F& r) {
+        if (lane_part())
+            fp_set_zero(r);
+        else
+            fp_set_one(r);
+    }
+};
+
 }  // namespace gm

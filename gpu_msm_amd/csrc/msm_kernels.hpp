@@ -99,8 +99,69 @@ template <class O>
 __device__ __forceinline__ void aff_set_inf(Aff<O>& p) {
     O::zero(p.x);
     O::zero(p.y);
-    reinterpret_cast<uint32_t*>(&p.x)[NL - 1] = INF_MARK;
+    uint32_t* xw = reinterpret_cast<uint32_t*>(&p.x);
+    xw[NL - 1] = INF_MARK;
+    if (sizeof(p.x) == 2 * sizeof(Fp)) xw[2 * NL - 1] = INF_MARK;
 }
+
+// ---------------------------------------------- storage of the hot kernels
+// Bases, buckets and partial sums are stored as full Aff / Xyzz records; the
+// G2 bucket kernels compute with Fp2LaneOps, two lanes per record, each lane
+// loading and storing its own component (Stor<O>::LPI lanes per item).
+template <class O>
+struct Stor {
+    using A = Aff<O>;
+    using X = Xyzz<O>;
+    static constexpr int LPI = 1;
+};
+template <>
+struct Stor<Fp2LaneOps> {
+    using A = Aff<Fp2Ops>;
+    using X = Xyzz<Fp2Ops>;
+    static constexpr int LPI = 2;
+};
+
+template <class O>
+__device__ __forceinline__ void ldp(Aff<O>& d, const Aff<O>* s) {
+    ld16(d, s);
+}
+template <class O>
+__device__ __forceinline__ void ldp(Xyzz<O>& d, const Xyzz<O>* s) {
+    ld16(d, s);
+}
+template <class O>
+__device__ __forceinline__ void stp(Xyzz<O>* d, const Xyzz<O>& v) {
+    st16(d, v);
+}
+__device__ __forceinline__ void ldp(Aff<Fp2LaneOps>& d, const Aff<Fp2Ops>* s) {
+    const uint32_t q = lane_part();
+    ld16(d.x, &s->x.c0 + q);
+    ld16(d.y, &s->y.c0 + q);
+}
+__device__ __forceinline__ void ldp(Xyzz<Fp2LaneOps>& d, const Xyzz<Fp2Ops>* s) {
+    const uint32_t q = lane_part();
+    ld16(d.X, &s->X.c0 + q);
+    ld16(d.Y, &s->Y.c0 + q);
+    ld16(d.ZZ, &s->ZZ.c0 + q);
+    ld16(d.ZZZ, &s->ZZZ.c0 + q);
+}
+__device__ __forceinline__ void stp(Xyzz<Fp2Ops>* d, const Xyzz<Fp2LaneOps>& v) {
+    const uint32_t q = lane_part();
+    st16(&d->X.c0 + q, v.X);
+    st16(&d->Y.c0 + q, v.Y);
+    st16(&d->ZZ.c0 + q, v.ZZ);
+    st16(&d->ZZZ.c0 + q, v.ZZZ);
+}
+
+// Compute policy of the bucket kernels for a curve's storage policy.
+template <class O>
+struct KernelOps {
+    using type = O;
+};
+template <>
+struct KernelOps<Fp2Ops> {
+    using type = Fp2LaneOps;
+};
 
 // ABI affine records (reference include/curve.h:13-25) -> device affine.
 template <class O>
@@ -178,11 +239,11 @@ __global__ void __launch_bounds__(64) k_gen_bases(Aff<O>* __restrict__ out, cons
 // those partials.
 template <class O>
 __global__ void __launch_bounds__(64) k_accumulate(const MsmDesc* __restrict__ desc, int count, uint32_t lanes,
-                                                   const Aff<O>* __restrict__ bases,
+                                                   const typename Stor<O>::A* __restrict__ bases,
                                                    const uint64_t* __restrict__ entries,
-                                                   const uint32_t* __restrict__ off, Xyzz<O>* __restrict__ buckets,
-                                                   Xyzz<O>* __restrict__ head, Xyzz<O>* __restrict__ tail) {
-    const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
+                                                   const uint32_t* __restrict__ off, typename Stor<O>::X* __restrict__ buckets,
+                                                   typename Stor<O>::X* __restrict__ head, typename Stor<O>::X* __restrict__ tail) {
+    const uint32_t l = (blockIdx.x * blockDim.x + threadIdx.x) / Stor<O>::LPI;
     if (l >= lanes) return;
     const MsmDesc& m = desc[GM_FIND(desc, count, lane_base, l)];
     const uint32_t e0 = off[m.bkt_base], e1 = off[m.bkt_base + m.nw * m.B];
@@ -198,25 +259,25 @@ __global__ void __launch_bounds__(64) k_accumulate(const MsmDesc* __restrict__ d
         const uint32_t g = (uint32_t)(e >> 32);
         if (g != cur) {
             if (first && off[cur] < j0)
-                st16(&head[l], acc);
+                stp(&head[l], acc);
             else
-                st16(&buckets[cur], acc);
+                stp(&buckets[cur], acc);
             xyzz_set_inf(acc);
             cur = g;
             first = false;
         }
         Aff<O> pt;
-        ld16(pt, &bases[(uint32_t)e & 0x7FFFFFFFu]);
+        ldp(pt, &bases[(uint32_t)e & 0x7FFFFFFFu]);
         if (aff_is_inf(pt)) continue;
         if ((uint32_t)e >> 31) O::neg(pt.y, pt.y);
         xyzz_add_aff<O>(acc, pt.x, pt.y);
     }
     if (first && off[cur] < j0)
-        st16(&head[l], acc);
+        stp(&head[l], acc);
     else if (off[cur + 1] > j1)
-        st16(&tail[l], acc);
+        stp(&tail[l], acc);
     else
-        st16(&buckets[cur], acc);
+        stp(&buckets[cur], acc);
 }
 
 // First and last accumulation lane whose chunk holds entries of bucket g
@@ -238,32 +299,32 @@ constexpr uint32_t HEAVY_SPAN = 8;
 template <class O>
 __global__ void __launch_bounds__(64) k_bucket_fixup(const MsmDesc* __restrict__ desc, int count, int nbk,
                                                      const uint32_t* __restrict__ off,
-                                                     const Xyzz<O>* __restrict__ head,
-                                                     const Xyzz<O>* __restrict__ tail, Xyzz<O>* __restrict__ buckets,
+                                                     const typename Stor<O>::X* __restrict__ head,
+                                                     const typename Stor<O>::X* __restrict__ tail, typename Stor<O>::X* __restrict__ buckets,
                                                      uint32_t* __restrict__ heavy, uint32_t* __restrict__ heavy_count) {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int g = (blockIdx.x * blockDim.x + threadIdx.x) / Stor<O>::LPI;
     if (g >= nbk) return;
     const uint32_t s = off[g], e = off[g + 1];
     Xyzz<O> acc;
     if (s == e) {
         xyzz_set_inf(acc);
-        st16(&buckets[g], acc);
+        stp(&buckets[g], acc);
         return;
     }
     uint32_t l0, l1;
     bucket_lanes(desc[GM_FIND(desc, count, bkt_base, (uint32_t)g)], off, s, e, l0, l1);
     if (l0 == l1) return;
     if (l1 - l0 > HEAVY_SPAN) {
-        heavy[atomicAdd(heavy_count, 1u)] = (uint32_t)g;
+        if (threadIdx.x % Stor<O>::LPI == 0) heavy[atomicAdd(heavy_count, 1u)] = (uint32_t)g;
         return;
     }
-    ld16(acc, &tail[l0]);
+    ldp(acc, &tail[l0]);
     for (uint32_t l = l0 + 1; l <= l1; ++l) {
         Xyzz<O> v;
-        ld16(v, &head[l]);
+        ldp(v, &head[l]);
         xyzz_add<O>(acc, acc, v);
     }
-    st16(&buckets[g], acc);
+    stp(&buckets[g], acc);
 }
 
 // One workgroup per queued heavy bucket (grid-stride over the queue): strided
@@ -271,9 +332,9 @@ __global__ void __launch_bounds__(64) k_bucket_fixup(const MsmDesc* __restrict__
 template <class O, int THREADS>
 __global__ void __launch_bounds__(THREADS) k_bucket_fixup_heavy(const MsmDesc* __restrict__ desc, int count,
                                                                 const uint32_t* __restrict__ off,
-                                                                const Xyzz<O>* __restrict__ head,
-                                                                const Xyzz<O>* __restrict__ tail,
-                                                                Xyzz<O>* __restrict__ buckets,
+                                                                const typename Stor<O>::X* __restrict__ head,
+                                                                const typename Stor<O>::X* __restrict__ tail,
+                                                                typename Stor<O>::X* __restrict__ buckets,
                                                                 const uint32_t* __restrict__ heavy,
                                                                 const uint32_t* __restrict__ heavy_count) {
     __shared__ Xyzz<O> sh[THREADS];
@@ -286,17 +347,17 @@ __global__ void __launch_bounds__(THREADS) k_bucket_fixup_heavy(const MsmDesc* _
         const uint32_t parts = l1 - l0 + 1;  // part 0 = tail[l0], part i = head[l0 + i]
         Xyzz<O> acc;
         xyzz_set_inf(acc);
-        for (uint32_t i = tid; i < parts; i += THREADS) {
+        for (uint32_t i = tid / Stor<O>::LPI; i < parts; i += THREADS / Stor<O>::LPI) {
             Xyzz<O> v;
             if (i == 0)
-                ld16(v, &tail[l0]);
+                ldp(v, &tail[l0]);
             else
-                ld16(v, &head[l0 + i]);
+                ldp(v, &head[l0 + i]);
             xyzz_add<O>(acc, acc, v);
         }
         sh[tid] = acc;
         __syncthreads();
-        for (int s = THREADS / 2; s > 0; s >>= 1) {
+        for (int s = THREADS / 2; s >= Stor<O>::LPI; s >>= 1) {
             if (tid < s) {
                 Xyzz<O> a = sh[tid], b = sh[tid + s];
                 xyzz_add<O>(a, a, b);
@@ -304,7 +365,7 @@ __global__ void __launch_bounds__(THREADS) k_bucket_fixup_heavy(const MsmDesc* _
             }
             __syncthreads();
         }
-        if (tid == 0) st16(&buckets[g], sh[0]);
+        if (tid < Stor<O>::LPI) stp(&buckets[g], sh[tid]);
         __syncthreads();
     }
 }
@@ -316,20 +377,20 @@ __global__ void __launch_bounds__(THREADS) k_bucket_fixup_heavy(const MsmDesc* _
 // as T + lo * S from running sums S = sum B_b, T = sum (b - lo + 1) B_b.
 template <class O>
 __global__ void __launch_bounds__(64) k_bucket_reduce(const MsmDesc* __restrict__ desc, int count, uint32_t seg_total,
-                                                      const Xyzz<O>* __restrict__ buckets,
-                                                      Xyzz<O>* __restrict__ segs) {
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+                                                      const typename Stor<O>::X* __restrict__ buckets,
+                                                      typename Stor<O>::X* __restrict__ segs) {
+    const uint32_t g = (blockIdx.x * blockDim.x + threadIdx.x) / Stor<O>::LPI;
     if (g >= seg_total) return;
     const MsmDesc& m = desc[GM_FIND(desc, count, seg_base, g)];
     const uint32_t local = g - m.seg_base;
     const uint32_t w = local / m.nseg, t = local - w * m.nseg;
-    const Xyzz<O>* bw = buckets + m.bkt_base + (size_t)w * m.B + (size_t)t * m.seg;
+    const typename Stor<O>::X* bw = buckets + m.bkt_base + (size_t)w * m.B + (size_t)t * m.seg;
     Xyzz<O> S, T;
     xyzz_set_inf(S);
     xyzz_set_inf(T);
     for (int b = (int)m.seg - 1; b >= 0; --b) {
         Xyzz<O> bk;
-        ld16(bk, &bw[b]);
+        ldp(bk, &bw[b]);
         xyzz_add<O>(S, S, bk);
         xyzz_add<O>(T, T, S);
     }
@@ -344,7 +405,7 @@ __global__ void __launch_bounds__(64) k_bucket_reduce(const MsmDesc* __restrict_
         }
         xyzz_add<O>(T, T, M);
     }
-    st16(&segs[g], T);
+    stp(&segs[g], T);
 }
 
 // Window sums in two short stages, so the serial depth stays small and the
@@ -354,10 +415,10 @@ __global__ void __launch_bounds__(64) k_bucket_reduce(const MsmDesc* __restrict_
 //                     in LDS;
 //   k_window_sum      one workgroup per (MSM, window): tree over its tiles.
 template <class O>
-__global__ void __launch_bounds__(64) k_window_partial(const MsmDesc* __restrict__ desc, int count,
-                                                       const Xyzz<O>* __restrict__ segs,
-                                                       Xyzz<O>* __restrict__ tiles) {
-    constexpr int T = 64;
+__global__ void __launch_bounds__(64 * Stor<O>::LPI) k_window_partial(const MsmDesc* __restrict__ desc, int count,
+                                                       const typename Stor<O>::X* __restrict__ segs,
+                                                       typename Stor<O>::X* __restrict__ tiles) {
+    constexpr int T = 64 * Stor<O>::LPI;
     __shared__ Xyzz<O> sh[T];
     const uint32_t gt = blockIdx.x;
     const int tid = threadIdx.x;
@@ -365,18 +426,18 @@ __global__ void __launch_bounds__(64) k_window_partial(const MsmDesc* __restrict
     const uint32_t ntile = (m.nseg + WSUM_TILE - 1) / WSUM_TILE;
     const uint32_t local = gt - m.tile_base;
     const uint32_t w = local / ntile, k = local - w * ntile;
-    const Xyzz<O>* sw = segs + m.seg_base + (size_t)w * m.nseg;
+    const typename Stor<O>::X* sw = segs + m.seg_base + (size_t)w * m.nseg;
     const uint32_t t1 = min((k + 1) * WSUM_TILE, m.nseg);
     Xyzz<O> acc;
     xyzz_set_inf(acc);
-    for (uint32_t t = k * WSUM_TILE + tid; t < t1; t += T) {
+    for (uint32_t t = k * WSUM_TILE + tid / Stor<O>::LPI; t < t1; t += T / Stor<O>::LPI) {
         Xyzz<O> v;
-        ld16(v, &sw[t]);
+        ldp(v, &sw[t]);
         xyzz_add<O>(acc, acc, v);
     }
     sh[tid] = acc;
     __syncthreads();
-    for (int s = T / 2; s > 0; s >>= 1) {
+    for (int s = T / 2; s >= Stor<O>::LPI; s >>= 1) {
         if (tid < s) {
             Xyzz<O> a = sh[tid], b = sh[tid + s];
             xyzz_add<O>(a, a, b);
@@ -384,30 +445,30 @@ __global__ void __launch_bounds__(64) k_window_partial(const MsmDesc* __restrict
         }
         __syncthreads();
     }
-    if (tid == 0) st16(&tiles[gt], sh[0]);
+    if (tid < Stor<O>::LPI) stp(&tiles[gt], sh[tid]);
 }
 
 template <class O, int THREADS>
 __global__ void __launch_bounds__(THREADS) k_window_sum(const MsmDesc* __restrict__ desc, int count,
-                                                        const Xyzz<O>* __restrict__ tiles,
-                                                        Xyzz<O>* __restrict__ wsum) {
+                                                        const typename Stor<O>::X* __restrict__ tiles,
+                                                        typename Stor<O>::X* __restrict__ wsum) {
     __shared__ Xyzz<O> sh[THREADS];
     const uint32_t gw = blockIdx.x;
     const int tid = threadIdx.x;
     const MsmDesc& m = desc[GM_FIND(desc, count, win_base, gw)];
     const uint32_t w = gw - m.win_base;
     const uint32_t ntile = (m.nseg + WSUM_TILE - 1) / WSUM_TILE;
-    const Xyzz<O>* tw = tiles + m.tile_base + (size_t)w * ntile;
+    const typename Stor<O>::X* tw = tiles + m.tile_base + (size_t)w * ntile;
     Xyzz<O> acc;
     xyzz_set_inf(acc);
-    for (uint32_t t = tid; t < ntile; t += THREADS) {
+    for (uint32_t t = tid / Stor<O>::LPI; t < ntile; t += THREADS / Stor<O>::LPI) {
         Xyzz<O> v;
-        ld16(v, &tw[t]);
+        ldp(v, &tw[t]);
         xyzz_add<O>(acc, acc, v);
     }
     sh[tid] = acc;
     __syncthreads();
-    for (int s = THREADS / 2; s > 0; s >>= 1) {
+    for (int s = THREADS / 2; s >= Stor<O>::LPI; s >>= 1) {
         if (tid < s) {
             Xyzz<O> a = sh[tid], b = sh[tid + s];
             xyzz_add<O>(a, a, b);
@@ -415,7 +476,7 @@ __global__ void __launch_bounds__(THREADS) k_window_sum(const MsmDesc* __restric
         }
         __syncthreads();
     }
-    if (tid == 0) st16(&wsum[gw], sh[0]);
+    if (tid < Stor<O>::LPI) stp(&wsum[gw], sh[tid]);
 }
 
 // Horner over window sums on the device, one lane per MSM; writes the
