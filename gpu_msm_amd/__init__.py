@@ -60,6 +60,8 @@ EXPORTS = [
     "gm_server_create", "gm_server_submit", "gm_server_wait", "gm_server_get_stats", "gm_server_inject_failures",
     "gm_server_destroy",
     "gm_set_profiling", "gm_profile_last", "gm_last_msm_info",
+    "gm_point_encoded_size", "gm_points_serialize", "gm_points_deserialize", "gm_points_check",
+    "gm_points_to_canonical", "gm_points_from_canonical",
 ]
 
 
@@ -127,6 +129,12 @@ def lib() -> ctypes.CDLL:
         "gm_set_profiling": ([u32, i], i),
         "gm_profile_last": ([u32, ctypes.POINTER(ctypes.c_float), i], i),
         "gm_last_msm_info": ([u32, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(ctypes.c_longlong)], i),
+        "gm_point_encoded_size": ([i, i], i),
+        "gm_points_serialize": ([i, vp, vp, i, i, u32], i),
+        "gm_points_deserialize": ([i, vp, vp, i, i, i, vp, u32], i),
+        "gm_points_check": ([i, vp, i, i, vp, u32], i),
+        "gm_points_to_canonical": ([i, vp, vp, i, u32], i),
+        "gm_points_from_canonical": ([i, vp, vp, i, u32], i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -281,6 +289,73 @@ def from_montgomery(curve: int, pts: np.ndarray) -> np.ndarray:
     size = pts.dtype.itemsize
     for k in range(len(pts)):
         f(out.ctypes.data + k * size, pts.ctypes.data + k * size)
+    return out
+
+
+# ---------------------------------------------------------- point formats
+# Batch conversions on the GPU (include/gpu_msm_ext.h, "point formats"): the
+# arkworks CanonicalSerialize layout of tfhe-zk-pok points (reference
+# rust-api/src/conversions.rs), validation, and canonical limbs.
+
+CHECK_ON_CURVE = 0x1
+CHECK_SUBGROUP = 0x2
+STATUS = {0: "ok", 1: "invalid encoding", 2: "not on the curve", 3: "not in the prime-order subgroup"}
+
+
+def _curve_dtype(curve: int):
+    return G1_DTYPE if curve == CURVE_G1 else G2_DTYPE
+
+
+def encoded_size(curve: int, compressed: bool = True) -> int:
+    return lib().gm_point_encoded_size(curve, int(compressed))
+
+
+def serialize_points(curve: int, points: np.ndarray, compressed: bool = True, gpu_index: int = 0) -> np.ndarray:
+    """ABI affine records (Montgomery) -> (n, size) uint8 encodings."""
+    pts = np.ascontiguousarray(points, dtype=_curve_dtype(curve))
+    size = encoded_size(curve, compressed)
+    out = np.zeros((len(pts), size), dtype=np.uint8)
+    rc = lib().gm_points_serialize(curve, _ptr(out), _ptr(pts), len(pts), int(compressed), gpu_index)
+    _check(rc, "point serialization")
+    return out
+
+
+def deserialize_points(curve: int, data: np.ndarray, compressed: bool = True,
+                       checks: int = CHECK_ON_CURVE | CHECK_SUBGROUP, gpu_index: int = 0):
+    """(n, size) uint8 encodings -> (ABI records in Montgomery form, int32
+    status per point). Invalid points come back as the point at infinity."""
+    size = encoded_size(curve, compressed)
+    raw = np.ascontiguousarray(data, dtype=np.uint8).reshape(-1, size)
+    out = np.zeros(len(raw), dtype=_curve_dtype(curve))
+    status = np.zeros(len(raw), dtype=np.int32)
+    rc = lib().gm_points_deserialize(curve, _ptr(out), _ptr(raw), len(raw), int(compressed), checks, _ptr(status),
+                                     gpu_index)
+    if rc < 0:
+        _check(rc, "point deserialization")
+    return out, status
+
+
+def check_points(curve: int, points: np.ndarray, checks: int = CHECK_ON_CURVE | CHECK_SUBGROUP,
+                 gpu_index: int = 0) -> np.ndarray:
+    pts = np.ascontiguousarray(points, dtype=_curve_dtype(curve))
+    status = np.zeros(len(pts), dtype=np.int32)
+    rc = lib().gm_points_check(curve, _ptr(pts), len(pts), checks, _ptr(status), gpu_index)
+    if rc < 0:
+        _check(rc, "point check")
+    return status
+
+
+def to_canonical(curve: int, points: np.ndarray, gpu_index: int = 0) -> np.ndarray:
+    pts = np.ascontiguousarray(points, dtype=_curve_dtype(curve))
+    out = np.zeros_like(pts)
+    _check(lib().gm_points_to_canonical(curve, _ptr(out), _ptr(pts), len(pts), gpu_index), "to canonical")
+    return out
+
+
+def from_canonical(curve: int, points: np.ndarray, gpu_index: int = 0) -> np.ndarray:
+    pts = np.ascontiguousarray(points, dtype=_curve_dtype(curve))
+    out = np.zeros_like(pts)
+    _check(lib().gm_points_from_canonical(curve, _ptr(out), _ptr(pts), len(pts), gpu_index), "from canonical")
     return out
 
 

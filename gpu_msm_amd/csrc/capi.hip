@@ -16,6 +16,7 @@
 #include "../../include/gpu_msm_ext.h"
 #include "../../include/rust_wrapper.h"
 #include "engine.hpp"
+#include "format.hpp"
 #include "host_combine.hpp"
 #include "host_field.hpp"
 #include "resident.hpp"
@@ -446,6 +447,54 @@ int batch_results(gm_batch* bt, void* results_affine) {
 
 }  // namespace
 
+// ------------------------------------------------------- point formats
+
+namespace {
+
+// One format / validation pass over n host records: upload, launch, download.
+// Returns the number of records with a non-zero status, or a negative code.
+template <class O>
+int fmt_run(int op, const void* host_in, size_t in_bytes, void* host_out, size_t out_bytes, int n, int arg,
+            int32_t* status, uint32_t gpu) {
+    DevCtx* c = get_ctx(gpu);
+    if (!c) return -2;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (hipSetDevice(c->dev) != hipSuccess) return -2;
+    const size_t o_out = gm::al_size(in_bytes), o_st = o_out + gm::al_size(out_bytes);
+    if (ensure_ws(c, o_st + gm::al_size((size_t)n * 4)) != hipSuccess) return -3;
+    uint8_t* d = c->ws;
+    int32_t* d_st = reinterpret_cast<int32_t*>(d + o_st);
+    if (hipMemcpyAsync(d, host_in, in_bytes, hipMemcpyHostToDevice, c->stream) != hipSuccess) return -4;
+    if (gm::launch_fmt<O>(c->stream, op, d, d + o_out, n, arg, d_st) != hipSuccess) return -4;
+    if (out_bytes && hipMemcpyAsync(host_out, d + o_out, out_bytes, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+        return -4;
+    std::vector<int32_t> st(status ? (size_t)n : 0);
+    if (status && hipMemcpyAsync(st.data(), d_st, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+        return -4;
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return -4;
+    int bad = 0;
+    for (int i = 0; i < (int)st.size(); ++i) {
+        status[i] = st[i];
+        bad += st[i] != 0;
+    }
+    return bad;
+}
+
+int fmt_curve(int curve, int op, const void* in, size_t in_rec, void* out, size_t out_rec, int n, int arg,
+              int32_t* status, uint32_t gpu) {
+    if (n < 0 || (n > 0 && !in)) return -1;
+    if (n == 0) return 0;
+    if (curve == GM_CURVE_G1)
+        return fmt_run<FpOps>(op, in, in_rec * n, out, out_rec * n, n, arg, status, gpu);
+    if (curve == GM_CURVE_G2)
+        return fmt_run<Fp2Ops>(op, in, in_rec * n, out, out_rec * n, n, arg, status, gpu);
+    return -1;
+}
+
+size_t aff_rec(int curve) { return curve == GM_CURVE_G1 ? sizeof(G1Point) : sizeof(G2Point); }
+
+}  // namespace
+
 // ================================================================= C ABI
 
 extern "C" {
@@ -807,6 +856,43 @@ int gm_profile_last(uint32_t gpu_index, float* stage_ms, int count) {
         stage_ms[i] = ms;
     }
     return 0;
+}
+
+int gm_point_encoded_size(int curve, int compressed) {
+    const int e = curve == GM_CURVE_G1 ? 56 : curve == GM_CURVE_G2 ? 112 : 0;
+    return compressed ? e : 2 * e;
+}
+
+int gm_points_serialize(int curve, void* out_bytes, const void* points, int n, int compressed, uint32_t gpu_index) {
+    const int sz = gm_point_encoded_size(curve, compressed);
+    if (sz == 0 || (n > 0 && !out_bytes)) return -1;
+    return fmt_curve(curve, gm::FMT_OP_SERIALIZE, points, aff_rec(curve), out_bytes, (size_t)sz, n,
+                     compressed ? 1 : 0, nullptr, gpu_index);
+}
+
+int gm_points_deserialize(int curve, void* points, const void* in_bytes, int n, int compressed, int checks,
+                          int32_t* status, uint32_t gpu_index) {
+    const int sz = gm_point_encoded_size(curve, compressed);
+    if (sz == 0 || (n > 0 && (!points || !status)) || (checks & ~3)) return -1;
+    return fmt_curve(curve, compressed ? gm::FMT_OP_DESERIALIZE_C : gm::FMT_OP_DESERIALIZE_U, in_bytes, (size_t)sz,
+                     points, aff_rec(curve), n, checks, status, gpu_index);
+}
+
+int gm_points_check(int curve, const void* points, int n, int checks, int32_t* status, uint32_t gpu_index) {
+    if ((n > 0 && !status) || (checks & ~3)) return -1;
+    return fmt_curve(curve, gm::FMT_OP_CHECK, points, aff_rec(curve), nullptr, 0, n, checks, status, gpu_index);
+}
+
+int gm_points_to_canonical(int curve, void* out, const void* points, int n, uint32_t gpu_index) {
+    if (n > 0 && !out) return -1;
+    return fmt_curve(curve, gm::FMT_OP_CANONICAL, points, aff_rec(curve), out, aff_rec(curve), n, 1, nullptr,
+                     gpu_index);
+}
+
+int gm_points_from_canonical(int curve, void* out, const void* points, int n, uint32_t gpu_index) {
+    if (n > 0 && !out) return -1;
+    return fmt_curve(curve, gm::FMT_OP_CANONICAL, points, aff_rec(curve), out, aff_rec(curve), n, 0, nullptr,
+                     gpu_index);
 }
 
 }  // extern "C"

@@ -227,6 +227,24 @@ constexpr uint32_t PM2_WORDS[14] = {0x3cdee0fb, 0x28c5e535, 0x200fc349, 0x65aad6
                                     0x8a02fe32, 0x0f75a64b, 0xbac71602, 0x824e6dc3, 0xe23acdee,
                                     0x56ee4528, 0xc573b5cc, 0x311c0026, 0xaab0aaa9};
 
+// (p + 1) / 4, (p - 1) / 2 and (p - 3) / 4 in 32-bit words, most significant
+// first: square root (p = 3 mod 4), Euler's criterion and the Fp2 square
+// root of Adj and Rodriguez-Henriquez (2012), Algorithm 9.
+constexpr uint32_t SQRT_WORDS[14] = {0x0f37b83e, 0xca31794d, 0x4803f0d2, 0x596ab590, 0x0025692d,
+                                     0xe280bf8c, 0x83dd6992, 0xeeb1c580, 0xa0939b70, 0xf88eb37b,
+                                     0x95bb914a, 0x315ced73, 0x0c470009, 0xaaac2aab};
+constexpr uint32_t EULER_WORDS[14] = {0x1e6f707d, 0x9462f29a, 0x9007e1a4, 0xb2d56b20, 0x004ad25b,
+                                      0xc5017f19, 0x07bad325, 0xdd638b01, 0x412736e1, 0xf11d66f7,
+                                      0x2b772294, 0x62b9dae6, 0x188e0013, 0x55585555};
+constexpr uint32_t P34_WORDS[14] = {0x0f37b83e, 0xca31794d, 0x4803f0d2, 0x596ab590, 0x0025692d,
+                                    0xe280bf8c, 0x83dd6992, 0xeeb1c580, 0xa0939b70, 0xf88eb37b,
+                                    0x95bb914a, 0x315ced73, 0x0c470009, 0xaaac2aaa};
+// (p - 1) / 2 in 28-bit limbs: canonical values above it are the "larger"
+// of each pair {y, -y}.
+constexpr uint32_t HALFL[NL] = {0x5585555, 0x8e00135, 0x9dae618, 0x229462b, 0x6f72b77, 0xe1f11d6,
+                                0x1412736, 0xdd638b0, 0x7bad325, 0x017f190, 0xad25bc5, 0x6b20004,
+                                0x1a4b2d5, 0x9a9007e, 0xd9462f2, 0x1e6f707};
+
 // a^(p-2) (Fermat inverse); 0 maps to 0.
 __device__ inline void fp_inv(Fp& r, const Fp& a) { fp_pow_words(r, a, PM2_WORDS, 14); }
 

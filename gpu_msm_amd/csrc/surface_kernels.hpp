@@ -19,14 +19,6 @@
 
 namespace gm {
 
-// (p + 1) / 4 and (p - 1) / 2 in 32-bit words, most significant first.
-constexpr uint32_t SQRT_WORDS[14] = {0x0f37b83e, 0xca31794d, 0x4803f0d2, 0x596ab590, 0x0025692d,
-                                     0xe280bf8c, 0x83dd6992, 0xeeb1c580, 0xa0939b70, 0xf88eb37b,
-                                     0x95bb914a, 0x315ced73, 0x0c470009, 0xaaac2aab};
-constexpr uint32_t EULER_WORDS[14] = {0x1e6f707d, 0x9462f29a, 0x9007e1a4, 0xb2d56b20, 0x004ad25b,
-                                      0xc5017f19, 0x07bad325, 0xdd638b01, 0x412736e1, 0xf11d66f7,
-                                      0x2b772294, 0x62b9dae6, 0x188e0013, 0x55585555};
-
 // ------------------------------------------------ Montgomery domain helpers
 
 __device__ __forceinline__ void to_mont(Fp& r, const Fp& a) {

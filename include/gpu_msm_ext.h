@@ -175,6 +175,47 @@ int gm_last_msm_info(uint32_t gpu_index, int* window, int* windows, long long* e
  * accumulate, fixup, bucket_reduce, window_sum]. */
 int gm_profile_last(uint32_t gpu_index, float* stage_ms, int count);
 
+/* ---------------------------------------------------------- point formats
+ * The wire format on the caller's side of the MSM (reference
+ * rust-api/src/conversions.rs:57-79 converts between the ABI records and
+ * tfhe-zk-pok's arkworks points). All pointers are HOST pointers; the work
+ * runs on the GPU in one batch. Points are ABI affine records in Montgomery
+ * form with field values below p, as every MSM entry point takes and returns.
+ *
+ * Encoding (the layout of ark-serialize's CanonicalSerialize for
+ * short-Weierstrass affine points): each Fp is its canonical integer in 56
+ * little-endian bytes, an Fp2 is c0 then c1; compressed = x, uncompressed =
+ * x then y; the last byte of the encoding carries the flags: 0x40 point at
+ * infinity (coordinates zero), 0x80 y is the larger of {y, -y} (integers for
+ * Fp; for Fp2, c1 decides and c0 breaks a tie). Sizes: G1 56 / 112 bytes,
+ * G2 112 / 224 bytes (compressed / uncompressed).
+ *
+ * Status per point (deserialize, check): 0 ok, 1 invalid encoding (a field
+ * element >= p, or both flag bits), 2 not on the curve (compressed: x^3 + b
+ * is not a square), 3 not in the prime-order subgroup. Points that fail are
+ * written as the point at infinity. The functions return the number of points
+ * with a non-zero status (0 = all valid), or -1 bad arguments, -2 no device,
+ * -3 allocation failure, -4 HIP error. No counterpart in the reference. */
+#define GM_CHECK_ON_CURVE 0x1 /* y^2 = x^3 + b with b = 1 (G1), 1 + i (G2) */
+#define GM_CHECK_SUBGROUP 0x2 /* r * P = O */
+
+int gm_point_encoded_size(int curve, int compressed);
+int gm_points_serialize(int curve, void* out_bytes, const void* points, int n, int compressed, uint32_t gpu_index);
+/* Compressed input is decompressed with a square root (then on the curve by
+ * construction); `checks` selects the validation of uncompressed input and
+ * the subgroup check of either. */
+int gm_points_deserialize(int curve, void* points, const void* in_bytes, int n, int compressed, int checks,
+                          int32_t* status, uint32_t gpu_index);
+/* Validation of ABI records (the reference has no is_on_curve; its curve
+ * constants name b = 4 and 4i, reference src/curve.cu:280-289, while the
+ * generators lie on y^2 = x^3 + 1 and y^2 = x^3 + (1 + i)). */
+int gm_points_check(int curve, const void* points, int n, int checks, int32_t* status, uint32_t gpu_index);
+/* Montgomery records <-> canonical standard-form records (the limbs
+ * from_sign_and_limbs takes and into_bigint returns); infinity records are
+ * written as x = y = 0 with the flag set. */
+int gm_points_to_canonical(int curve, void* out, const void* points, int n, uint32_t gpu_index);
+int gm_points_from_canonical(int curve, void* out, const void* points, int n, uint32_t gpu_index);
+
 #ifdef __cplusplus
 }
 #endif

@@ -221,3 +221,53 @@ def gen_scalars_modr(seed: int, n: int):
         if s < R_ORDER:
             out.append(s)
     return out
+
+
+# ------------------------------------------------ point encoding (wire format)
+# The layout of ark-serialize's CanonicalSerialize for short-Weierstrass
+# affine points (the format tfhe-zk-pok uses; reference
+# rust-api/src/conversions.rs converts to those points): canonical integers,
+# 56 little-endian bytes per Fp (446 bits leave the top two bits of the last
+# byte free), Fp2 as c0 then c1; compressed = x, uncompressed = x then y; flag
+# 0x40 infinity, 0x80 "y is the larger of {y, -y}" in the encoding's last byte.
+# Written from that description, independently of the device code.
+
+HALF = (P - 1) // 2
+
+
+def fp_bytes(v: int) -> bytes:
+    return int(v).to_bytes(56, "little")
+
+
+def larger_g1(y: int) -> bool:
+    return y > HALF
+
+
+def larger_g2(y) -> bool:
+    return y[1] > HALF if y[1] else y[0] > HALF
+
+
+def encode_point(curve: str, pt, compressed: bool) -> bytes:
+    """pt: standard-form coordinates (ints for G1, pairs for G2) or None."""
+    esz = 56 if curve == "g1" else 112
+    size = esz if compressed else 2 * esz
+    if pt is None:
+        b = bytearray(size)
+        b[-1] |= 0x40
+        return bytes(b)
+    x, y = pt
+    enc = (lambda v: fp_bytes(v)) if curve == "g1" else (lambda v: fp_bytes(v[0]) + fp_bytes(v[1]))
+    b = bytearray(enc(x) if compressed else enc(x) + enc(y))
+    if (larger_g1(y) if curve == "g1" else larger_g2(y)):
+        b[-1] |= 0x80
+    return bytes(b)
+
+
+def is_square_fp(a: int) -> bool:
+    return a % P == 0 or pow(a, (P - 1) // 2, P) == 1
+
+
+def is_square_fp2(a) -> bool:
+    """a is a square in Fp2 = Fp[i]/(i^2 + 1) iff its norm a0^2 + a1^2 is a
+    square in Fp."""
+    return is_square_fp((a[0] * a[0] + a[1] * a[1]) % P)
