@@ -2,7 +2,8 @@
 //
 // Bucket accumulators use extended Jacobian "XYZZ" coordinates
 // (x = X/ZZ, y = Y/ZZZ, ZZ^3 = ZZZ^2), whose mixed addition costs 8M + 2S and
-// needs no curve constant. The point at infinity is ZZ = ZZZ = 0. Results are
+// needs no curve constant. Each formula's final Y3 = A B - C D is one lazily
+// reduced difference of products (O::mul_sub). The point at infinity is ZZ = ZZZ = 0. Results are
 // compared only after affine normalisation, where every representation of a
 // point gives the same canonical (x, y) as the reference's standard-projective
 // formulas (reference src/curve.cu:444-778).
@@ -72,9 +73,7 @@ __device__ __forceinline__ void xyzz_dbl_aff(Xyzz<O>& r, const typename O::F& x,
     O::sub(t, t, S);
     O::sub(r.X, t, S);   // X3 = M^2 - 2S
     O::sub(t, S, r.X);
-    O::mul(t, M, t);
-    O::mul(S, W, y);
-    O::sub(r.Y, t, S);   // Y3 = M(S - X3) - W y
+    O::mul_sub(r.Y, M, t, W, y);  // Y3 = M(S - X3) - W y
     r.ZZ = V;
     r.ZZZ = W;
 }
@@ -97,13 +96,11 @@ __device__ __forceinline__ void xyzz_dbl(Xyzz<O>& r, const Xyzz<O>& p) {
     O::add(M, M, t);
     O::mul(r.ZZ, V, p.ZZ);
     O::mul(r.ZZZ, W, p.ZZZ);
-    O::mul(W, W, p.Y);
     O::sqr(t, M);
     O::sub(t, t, S);
     O::sub(r.X, t, S);
     O::sub(t, S, r.X);
-    O::mul(t, M, t);
-    O::sub(r.Y, t, W);
+    O::mul_sub(r.Y, M, t, W, p.Y);  // p.Y is read before r.Y is written (r may alias p)
 }
 
 // acc += (x2, y2) affine (madd-2008-s). Handles acc = O, equal points
@@ -141,9 +138,7 @@ __device__ __forceinline__ void xyzz_add_aff(Xyzz<O>& acc, const typename O::F& 
     O::sub(t, t, Q);
     O::sub(acc.X, t, Q);     // X3 = R^2 - PPP - 2Q
     O::sub(t, Q, acc.X);
-    O::mul(t, R, t);
-    O::mul(PPP, acc.Y, PPP);
-    O::sub(acc.Y, t, PPP);   // Y3 = R(Q - X3) - Y1 PPP
+    O::mul_sub(acc.Y, R, t, acc.Y, PPP);  // Y3 = R(Q - X3) - Y1 PPP, one reduction
 }
 
 // r = p + q, both XYZZ (add-2008-s). r may alias p or q.
@@ -180,14 +175,12 @@ __device__ __forceinline__ void xyzz_add(Xyzz<O>& r, const Xyzz<O>& p, const Xyz
     O::mul(r.ZZ, t, PP);
     O::mul(t, p.ZZZ, q.ZZZ);
     O::mul(r.ZZZ, t, PPP);
-    O::mul(S1, S1, PPP);
     O::sqr(t, R);
     O::sub(t, t, PPP);
     O::sub(t, t, Q);
     O::sub(r.X, t, Q);
     O::sub(t, Q, r.X);
-    O::mul(t, R, t);
-    O::sub(r.Y, t, S1);
+    O::mul_sub(r.Y, R, t, S1, PPP);
 }
 
 // r = -p

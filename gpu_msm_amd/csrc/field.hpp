@@ -248,6 +248,95 @@ constexpr uint32_t HALFL[NL] = {0x5585555, 0x8e00135, 0x9dae618, 0x229462b, 0x6f
 // a^(p-2) (Fermat inverse); 0 maps to 0.
 __device__ inline void fp_inv(Fp& r, const Fp& a) { fp_pow_words(r, a, PM2_WORDS, 14); }
 
+// Signed Montgomery reduction of a sum of limb products, the core of the
+// lazily reduced products below: value = sum over the T terms of x_t * y_t,
+// with y limbs signed (a negated operand makes a difference of products).
+// Every product is one signed 32 x 32 -> 64 mad into the column sum; limbs
+// are below 2^28 in magnitude, so T = 4 terms (64 products per column) plus
+// 16 reduction products stay below 2^63. Returns the reduced value in
+// 28-bit limbs with its top limb signed (in the last word, sign carried by
+// the int64 returned as `top`).
+template <int T>
+__host__ __device__ __forceinline__ void redc_signed(uint32_t (&t)[NL], int64_t& top, const Fp* const (&x)[T],
+                                            const int32_t (*const (&y)[T])[NL]) {
+    uint32_t m[NL];
+    int64_t acc = 0;
+#pragma unroll
+    for (int k = 0; k < NL; ++k) {
+#pragma unroll
+        for (int i = 0; i <= k; ++i)
+#pragma unroll
+            for (int u = 0; u < T; ++u) acc += (int64_t)(int32_t)x[u]->v[i] * (*y[u])[k - i];
+#pragma unroll
+        for (int i = 0; i < k; ++i) acc += (int64_t)((uint64_t)m[i] * PL[k - i]);
+        m[k] = ((uint32_t)acc * PINV28) & MASK28;
+        acc += (int64_t)((uint64_t)m[k] * PL[0]);
+        acc >>= 28;  // arithmetic: the low 28 bits are zero
+    }
+#pragma unroll
+    for (int k = NL; k < 2 * NL - 1; ++k) {
+#pragma unroll
+        for (int i = k - NL + 1; i < NL; ++i) {
+#pragma unroll
+            for (int u = 0; u < T; ++u) acc += (int64_t)(int32_t)x[u]->v[i] * (*y[u])[k - i];
+            acc += (int64_t)((uint64_t)m[i] * PL[k - i]);
+        }
+        t[k - NL] = (uint32_t)acc & MASK28;
+        acc >>= 28;
+    }
+    top = acc;
+}
+
+// Brings a reduced value in (-2p, 3p) (28-bit limbs 0..14 and a signed top
+// limb) to [0, 2p): add 2p when negative, then subtract 2p unless that
+// borrows. `sub_pass` = false skips the second step for values known to be
+// below 2p.
+__host__ __device__ __forceinline__ void redc_fold(Fp& r, uint32_t (&t)[NL], int64_t top, bool sub_pass) {
+    const uint32_t msk = top < 0 ? 0xFFFFFFFFu : 0u;
+    int32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < NL - 1; ++i) {
+        const int32_t v = (int32_t)t[i] + (int32_t)(P2L[i] & msk) + c;
+        t[i] = (uint32_t)v & MASK28;
+        c = v >> 28;
+    }
+    t[NL - 1] = (uint32_t)((int32_t)top + (int32_t)(P2L[NL - 1] & msk) + c);
+    if (!sub_pass) {
+#pragma unroll
+        for (int i = 0; i < NL; ++i) r.v[i] = t[i];
+        return;
+    }
+    uint32_t d[NL];
+    int32_t br = 0;
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+        const int32_t v = (int32_t)t[i] - (int32_t)P2L[i] + br;
+        d[i] = (uint32_t)v & MASK28;
+        br = v >> 28;
+    }
+    const bool keep = br < 0;
+#pragma unroll
+    for (int i = 0; i < NL; ++i) r.v[i] = keep ? t[i] : d[i];
+}
+
+// r = (a b - c d) R^{-1} mod p with one reduction. Inputs in [0, 2p): the
+// difference lies in (-4p^2, 4p^2), so the reduced value is in (-p, 2p) and
+// one conditional addition of 2p lands in [0, 2p).
+__host__ __device__ __forceinline__ void fp_mul_sub(Fp& r, const Fp& a, const Fp& b, const Fp& c, const Fp& d) {
+    int32_t yb[NL], nd[NL];
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+        yb[i] = (int32_t)b.v[i];
+        nd[i] = -(int32_t)d.v[i];
+    }
+    const Fp* const x[2] = {&a, &c};
+    const int32_t(*const y[2])[NL] = {&yb, &nd};
+    uint32_t t[NL];
+    int64_t top;
+    redc_signed<2>(t, top, x, y);
+    redc_fold(r, t, top, false);
+}
+
 // ------------------------------------------------------------------ Fp2
 // Fp2 = Fp[i]/(i^2 + 1) (non-residue -1 as in reference include/fp2.h:7).
 
@@ -387,6 +476,10 @@ struct FpOps {
     __device__ static __forceinline__ void neg(F& r, const F& a) { fp_neg(r, a); }
     __device__ static __forceinline__ void mul(F& r, const F& a, const F& b) { fp_mul(r, a, b); }
     __device__ static __forceinline__ void sqr(F& r, const F& a) { fp_sqr(r, a); }
+    // r = a b - c d
+    __device__ static __forceinline__ void mul_sub(F& r, const F& a, const F& b, const F& c, const F& d) {
+        fp_mul_sub(r, a, b, c, d);
+    }
     __device__ static __forceinline__ bool is_zero(const F& a) { return fp_is_zero(a); }
     __device__ static __forceinline__ bool is_raw_zero(const F& a) { return fp_is_raw_zero(a); }
     __device__ static __forceinline__ void canon(F& r, const F& a) { fp_canon(r, a); }
@@ -404,6 +497,12 @@ struct Fp2Ops {
     __device__ static __forceinline__ void neg(F& r, const F& a) { fp2_neg(r, a); }
     __device__ static __forceinline__ void mul(F& r, const F& a, const F& b) { fp2_mul(r, a, b); }
     __device__ static __forceinline__ void sqr(F& r, const F& a) { fp2_sqr(r, a); }
+    __device__ static __forceinline__ void mul_sub(F& r, const F& a, const F& b, const F& c, const F& d) {
+        F t;
+        fp2_mul(t, c, d);
+        fp2_mul(r, a, b);
+        fp2_sub(r, r, t);
+    }
     __device__ static __forceinline__ bool is_zero(const F& a) { return fp2_is_zero(a); }
     __device__ static __forceinline__ bool is_raw_zero(const F& a) { return fp2_is_raw_zero(a); }
     __device__ static __forceinline__ void canon(F& r, const F& a) { fp2_canon(r, a); }
@@ -444,65 +543,49 @@ __device__ __forceinline__ void fp_pair_swap(Fp& r, const Fp& a) {
     for (int i = 0; i < NL; ++i) r.v[i] = pair_swap(a.v[i]);
 }
 
-// Own component of (a0 + a1 i)(b0 + b1 i); ao / bo are the partner's parts.
-__device__ __forceinline__ void fp2_lane_mul(Fp& r, const Fp& a, const Fp& ao, const Fp& b, const Fp& bo) {
+// Signed limbs of the operands a lane multiplies for its own component of
+// (a0 + a1 i)(b0 + b1 i) = (a0 b0 - a1 b1) + (a1 b0 + a0 b1) i, as
+// a y + ao w with ao the partner's part of a:
+//   part 0: y = b0 (own), w = -b1 (partner);  part 1: y = b0 (partner), w = +b1 (own).
+// `neg` negates both (for a subtracted product).
+__device__ __forceinline__ void lane_operands(int32_t (&y)[NL], int32_t (&w)[NL], const Fp& b, const Fp& bo,
+                                              bool neg) {
     const bool q = lane_part() != 0;
-    uint32_t y[NL];
-    int32_t w[NL];
 #pragma unroll
     for (int i = 0; i < NL; ++i) {
-        y[i] = q ? bo.v[i] : b.v[i];
-        const int32_t v = (int32_t)(q ? b.v[i] : bo.v[i]);
-        w[i] = q ? v : -v;
+        const int32_t yv = (int32_t)(q ? bo.v[i] : b.v[i]);
+        const int32_t wv = (int32_t)(q ? b.v[i] : bo.v[i]);
+        y[i] = neg ? -yv : yv;
+        w[i] = (q != neg) ? wv : -wv;
     }
-    uint32_t m[NL];
-    int64_t acc = 0;
-#pragma unroll
-    for (int k = 0; k < NL; ++k) {
-#pragma unroll
-        for (int i = 0; i <= k; ++i) {
-            acc += (int64_t)(int32_t)a.v[i] * (int32_t)y[k - i];
-            acc += (int64_t)(int32_t)ao.v[i] * w[k - i];
-        }
-#pragma unroll
-        for (int i = 0; i < k; ++i) acc += (int64_t)((uint64_t)m[i] * PL[k - i]);
-        m[k] = ((uint32_t)acc * PINV28) & MASK28;
-        acc += (int64_t)((uint64_t)m[k] * PL[0]);
-        acc >>= 28;
-    }
+}
+
+// Own component of a b; ao / bo are the partner's parts.
+__device__ __forceinline__ void fp2_lane_mul(Fp& r, const Fp& a, const Fp& ao, const Fp& b, const Fp& bo) {
+    int32_t y[NL], w[NL];
+    lane_operands(y, w, b, bo, false);
+    const Fp* const x[2] = {&a, &ao};
+    const int32_t(*const yy[2])[NL] = {&y, &w};
     uint32_t t[NL];
-#pragma unroll
-    for (int k = NL; k < 2 * NL - 1; ++k) {
-#pragma unroll
-        for (int i = k - NL + 1; i < NL; ++i) {
-            acc += (int64_t)(int32_t)a.v[i] * (int32_t)y[k - i];
-            acc += (int64_t)(int32_t)ao.v[i] * w[k - i];
-            acc += (int64_t)((uint64_t)m[i] * PL[k - i]);
-        }
-        t[k - NL] = (uint32_t)acc & MASK28;
-        acc >>= 28;
-    }
-    // add 2p when negative (part 0), then subtract 2p unless that borrows
-    const uint32_t msk = acc < 0 ? 0xFFFFFFFFu : 0u;
-    int32_t c = 0;
-#pragma unroll
-    for (int i = 0; i < NL - 1; ++i) {
-        const int32_t v = (int32_t)t[i] + (int32_t)(P2L[i] & msk) + c;
-        t[i] = (uint32_t)v & MASK28;
-        c = v >> 28;
-    }
-    t[NL - 1] = (uint32_t)((int32_t)acc + (int32_t)(P2L[NL - 1] & msk) + c);
-    uint32_t d[NL];
-    int32_t br = 0;
-#pragma unroll
-    for (int i = 0; i < NL; ++i) {
-        const int32_t v = (int32_t)t[i] - (int32_t)P2L[i] + br;
-        d[i] = (uint32_t)v & MASK28;
-        br = v >> 28;
-    }
-    const bool keep = br < 0;
-#pragma unroll
-    for (int i = 0; i < NL; ++i) r.v[i] = keep ? t[i] : d[i];
+    int64_t top;
+    redc_signed<2>(t, top, x, yy);
+    // part 0 lies in (-p, 2p), part 1 in [0, 3p)
+    redc_fold(r, t, top, true);
+}
+
+// Own component of a b - c d with one reduction: both parts lie in
+// (-8p^2, 8p^2) before and in (-2p, 3p) after the reduction.
+__device__ __forceinline__ void fp2_lane_mul_sub(Fp& r, const Fp& a, const Fp& ao, const Fp& b, const Fp& bo,
+                                                 const Fp& c, const Fp& co, const Fp& d, const Fp& dox) {
+    int32_t y0[NL], w0[NL], y1[NL], w1[NL];
+    lane_operands(y0, w0, b, bo, false);
+    lane_operands(y1, w1, d, dox, true);
+    const Fp* const x[4] = {&a, &ao, &c, &co};
+    const int32_t(*const yy[4])[NL] = {&y0, &w0, &y1, &w1};
+    uint32_t t[NL];
+    int64_t top;
+    redc_signed<4>(t, top, x, yy);
+    redc_fold(r, t, top, true);
 }
 
 struct Fp2LaneOps {
@@ -522,6 +605,14 @@ struct Fp2LaneOps {
         Fp ao;
         fp_pair_swap(ao, a);
         fp2_lane_mul(r, a, ao, a, ao);
+    }
+    __device__ static __forceinline__ void mul_sub(F& r, const F& a, const F& b, const F& c, const F& d) {
+        Fp ao, bo, co, dx;
+        fp_pair_swap(ao, a);
+        fp_pair_swap(bo, b);
+        fp_pair_swap(co, c);
+        fp_pair_swap(dx, d);
+        fp2_lane_mul_sub(r, a, ao, b, bo, c, co, d, dx);
     }
     __device__ static __forceinline__ bool is_zero(const F& a) {
         const uint32_t z = fp_is_zero(a) ? 1u : 0u;
