@@ -2,7 +2,7 @@
 (curve, log2 N, batch size) on homogeneous batches, the best-plan table, and
 the planner MLP trained on the sweeps.
 
-    python -m gpu_msm_amd.autotune [--out gpu_msm_amd/tuning/sweep.json] [--quick]
+    python -m gpu_msm_amd.autotune [--out gpu_msm_amd/tuning/sweep.json] [--quick] [--train-only]
 
 Each cell times `batch` MSMs of size N in one pipeline run (gm_msm_batch,
 host window combine) for every candidate plan; the per-MSM time is the run
@@ -52,7 +52,10 @@ def time_cell(gm, bases, scalars, pool_n: int, curve: int, log2n: int, batch: in
     return best * 1e3 / batch
 
 
-def sweep(curves=(1, 2), log2ns=range(8, 21), batches=(1, 8, 64, 256), reps: int = 2, log=print):
+def sweep(curves=(1, 2), log2ns=range(8, 21), batches=(1, 8, 64, 256), reps: int = 2, log=print,
+          on_cell=None):
+    """Timings of every candidate plan per cell; `on_cell(records)` is called
+    after each cell (the caller can checkpoint partial sweeps)."""
     import gpu_msm_amd as gm
 
     rng = np.random.default_rng(11)
@@ -75,6 +78,8 @@ def sweep(curves=(1, 2), log2ns=range(8, 21), batches=(1, 8, 64, 256), reps: int
                 best = min(cell, key=lambda r: r["ms_per_msm"])
                 log(f"autotune: curve {curve} N=2^{lg} batch {b}: best c={best['c']} seg={best['seg']} "
                     f"{best['ms_per_msm']:.4f} ms/MSM over {len(cell)} plans")
+                if on_cell:
+                    on_cell(records)
         bases.free()
     sc.free()
     return records
@@ -93,10 +98,12 @@ def best_table(records):
 
 def training_set(records, tol: float = 0.02):
     """One sample per plan within `tol` of its cell's best time, so flat
-    optima give averaged (not arbitrary) targets."""
+    optima give averaged (not arbitrary) targets. Widths with a sparse top
+    window are left out: the planner never picks them."""
     cells = {}
     for r in records:
-        cells.setdefault((r["curve"], r["log2n"], r["batch"]), []).append(r)
+        if pl.window_ok(r["c"]):
+            cells.setdefault((r["curve"], r["log2n"], r["batch"]), []).append(r)
     xs, ts = [], []
     for (cv, lg, b), rs in sorted(cells.items()):
         t0 = min(q["ms_per_msm"] for q in rs)
@@ -113,16 +120,25 @@ def main() -> None:
     ap.add_argument("--mlp-out", default=os.path.join(pl.TUNING_DIR, "mlp.json"))
     ap.add_argument("--quick", action="store_true")
     ap.add_argument("--epochs", type=int, default=4000)
+    ap.add_argument("--train-only", action="store_true", help="retrain the MLP on the existing sweep file")
     args = ap.parse_args()
     log = lambda s: print(s, file=sys.stderr, flush=True)  # noqa: E731
-    if args.quick:
-        recs = sweep(curves=(1,), log2ns=(8, 12, 16), batches=(1, 8), reps=1, log=log)
+    os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
+
+    def save(recs):
+        with open(args.out, "w") as f:
+            json.dump({"records": recs, "best": best_table(recs)}, f, indent=0)
+
+    if args.train_only:
+        with open(args.out) as f:
+            recs = json.load(f)["records"]
+    elif args.quick:
+        recs = sweep(curves=(1,), log2ns=(8, 12, 16), batches=(1, 8), reps=1, log=log, on_cell=save)
     else:
-        recs = sweep(log=log)
+        recs = sweep(log=log, on_cell=save)
     table = best_table(recs)
-    os.makedirs(os.path.dirname(args.out), exist_ok=True)
-    with open(args.out, "w") as f:
-        json.dump({"records": recs, "best": table}, f, indent=0)
+    if not args.train_only:
+        save(recs)
     x, t = training_set(recs)
     params, curve = pl.mlp_train(x, t, epochs=args.epochs)
     with open(args.mlp_out, "w") as f:

@@ -105,6 +105,7 @@ struct Plan {
 size_t xyzz_bytes(int curve);
 size_t aff_bytes(int curve);
 int choose_window(int curve, int n, int bits);
+bool window_ok(int bits, int c);
 int choose_seg(int B);
 // Fills p (and p.host_desc[0..count)) for a batch; false on invalid specs.
 // exact_scan_bytes = false bounds the scan's temporary storage instead of

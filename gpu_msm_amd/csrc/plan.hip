@@ -16,22 +16,32 @@ size_t al_size(size_t x) { return (x + 255) & ~(size_t)255; }
 
 static int windows_for(int bits, int c) { return (bits + 1 + c - 1) / c; }
 
-// Buckets per bucket-reduce lane: short segments keep enough lanes in flight
-// (the reduction is latency-bound); the price is one small scalar multiple
-// per segment.
+// A width whose top window keeps fewer than 4/5 of its c bits sends every
+// point of that window to a handful of buckets (c = 11 for 299-bit scalars
+// leaves it 3 bits: 2^14 points land in 4 buckets). The bucket phase then
+// sums hundreds of chunk partials per bucket on a long serial chain, which
+// the sweeps measure at up to 1.6x the time of a neighbouring width.
+bool window_ok(int bits, int c) {
+    const int top = bits + 1 - (windows_for(bits, c) - 1) * c;
+    return 5 * top >= 4 * c;
+}
+
+// Buckets per bucket-reduce lane assumed by the window cost model (the plan
+// itself sizes segments for the whole batch, see make_plan).
 int choose_seg(int B) { return B < 8 ? B : 8; }
 
 // Relative cost in Fp-multiplication equivalents: mixed adds of the bucket
-// phase (10 products each, 3x for Fp2), the running-sum adds and per-segment
-// scalar multiples of the bucket reduction, and a per-(window, point) charge
-// for recoding and sorting.
+// phase (10 products each, 3x for Fp2), the running-sum adds (two full adds
+// per bucket, weighted up for their lower occupancy) and per-segment scalar
+// multiples of the bucket reduction, and a per-(window, point) charge for
+// recoding and sorting.
 static double plan_cost(int curve, int n, int bits, int c) {
     const int nw = windows_for(bits, c);
     const double B = (double)(1 << (c - 1));
     const double seg = choose_seg(1 << (c - 1));
     const double fm = curve == CURVE_G1 ? 1.0 : 3.0;
     const double accum = (double)nw * n * 10.5 * fm;
-    const double reduce = (double)nw * (B * 28.0 + (B / seg) * 18.0 * c) * fm;
+    const double reduce = (double)nw * (B * 56.0 + (B / seg) * 18.0 * c) * fm;
     const double sort = (double)nw * n * 0.6;
     return accum + reduce + sort;
 }
@@ -40,6 +50,7 @@ int choose_window(int curve, int n, int bits) {
     int best = 2;
     double bc = 1e300;
     for (int c = 2; c <= 20; ++c) {
+        if (!window_ok(bits, c)) continue;
         const double v = plan_cost(curve, n, bits, c);
         if (v < bc) {
             bc = v;
@@ -59,6 +70,28 @@ bool make_plan(Plan& p, MsmDesc* d, int curve, int L, int bits, const MsmSpec* s
     p.bits = bits;
     p.count = count;
     p.host_desc = d;
+    // Buckets per bucket-reduction lane for the MSMs that leave it to the
+    // plan: the smallest power of two >= 4 that keeps the batch's reduction
+    // within REDUCE_THREADS threads (G2 runs two threads per lane), at most
+    // 16. Fewer, longer lanes leave SIMDs idle; more, shorter ones add rounds
+    // and the per-segment scalar multiples. The sweeps put the optimum at this
+    // size for both curves (seg 8 for G1 and 16 for G2 at N = 2^20, 4 for
+    // small MSMs).
+    constexpr uint64_t REDUCE_THREADS = 48 * 1024;
+    const uint64_t lpi = curve == CURVE_G2 ? 2 : 1;
+    uint64_t auto_bkt = 0, fixed_thr = 0;
+    for (int j = 0; j < count; ++j) {
+        const MsmSpec& s = specs[j];
+        const int c = s.c > 0 ? s.c : choose_window(curve, s.n, bits);
+        if (c < 2 || c > 24) return false;
+        const uint64_t B = 1ull << (c - 1), nbk = B * (uint64_t)windows_for(bits, c);
+        if (s.seg > 0)
+            fixed_thr += nbk / std::min<uint64_t>(B, (uint64_t)s.seg) * lpi;
+        else
+            auto_bkt += nbk;
+    }
+    uint32_t auto_seg = 4;
+    while (auto_seg < 16 && fixed_thr + auto_bkt * lpi / auto_seg > REDUCE_THREADS) auto_seg <<= 1;
     uint64_t pt = 0, rank = 0, bkt = 0, seg = 0, win = 0, tile = 0;
     for (int j = 0; j < count; ++j) {
         const MsmSpec& s = specs[j];
@@ -67,7 +100,7 @@ bool make_plan(Plan& p, MsmDesc* d, int curve, int L, int bits, const MsmSpec* s
         if (c < 2 || c > 24) return false;
         const int nw = windows_for(bits, c);
         const int B = 1 << (c - 1);
-        int sg = s.seg > 0 ? s.seg : choose_seg(B);
+        int sg = s.seg > 0 ? s.seg : (int)auto_seg;
         if (sg > B) sg = B;
         if ((sg & (sg - 1)) != 0) return false;  // power of two dividing B
         MsmDesc& m = d[j];

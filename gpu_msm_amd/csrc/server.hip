@@ -109,7 +109,14 @@ struct HostMlp {
         }
         const double cc = y[0] * 4.0 + 12.0, ls = y[1] * 1.5 + 3.0;
         if (!std::isfinite(cc) || !std::isfinite(ls) || cc < 2.0 || cc > 21.0) return;  // heuristic fallback
-        c = std::min(20, std::max(3, (int)lround(cc)));
+        // c inside the sweeps' candidate band around the heuristic, at the
+        // nearest width whose top window is not sparse
+        const int h = gm::choose_window(curve, std::max(n, 1), 299);
+        const int lo = std::max(3, h - 3), hi = std::min(20, h + 2);
+        const double ct = std::min((double)hi, std::max((double)lo, cc));
+        c = h;
+        for (int k = lo; k <= hi; ++k)
+            if (gm::window_ok(299, k) && std::fabs(k - ct) < std::fabs(c - ct)) c = k;
         seg = 1 << (int)lround(std::min(5.0, std::max(1.0, ls)));
         seg = std::min(seg, 1 << (c - 1));
     }

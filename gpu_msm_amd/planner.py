@@ -9,8 +9,10 @@ segs)``:
   batch size), nearest neighbour in (log2 N, log2 batch).
 - ``MlpPlanner``: an FP32 MLP (inputs log2 N, batch size, curve; two hidden
   layers of 32) trained on the autotuner's sweeps by the library's own HIP
-  kernels. Its outputs are rounded to the nearest valid (c, seg); a
-  non-finite or out-of-range prediction falls back to the heuristic.
+  kernels. Its outputs are rounded to the nearest valid (c, seg): c is kept
+  inside the sweeps' candidate band (heuristic c - 3 .. + 2, where the
+  network was trained) and off widths with a sparse top window; a non-finite
+  prediction falls back to the heuristic.
 
 Every planner only changes how the work is split, never the result: any
 valid (c, seg) gives the same bit-exact MSM.
@@ -63,6 +65,26 @@ def decode(y: np.ndarray):
         seg = min(seg, 1 << (cc - 1))
         out.append((cc, seg))
     return out
+
+
+def window_ok(c: int, bits: int = 299) -> bool:
+    """Same rule as the library's window_ok: the top window keeps at least
+    4/5 of its c bits (a sparse top window sends a whole window to a few
+    buckets and serialises the bucket phase)."""
+    nw = (bits + c) // c
+    return 5 * (bits + 1 - (nw - 1) * c) >= 4 * c
+
+
+def snap_window(c: float, h: int, bits: int = 299) -> int:
+    """Predicted width -> the nearest width in the sweeps' candidate band
+    [h - 3, h + 2] around the heuristic h whose top window is not sparse."""
+    lo, hi = max(C_MIN, h - 3), min(C_MAX, h + 2)
+    ct = min(max(float(c), lo), hi)
+    best = h
+    for k in range(lo, hi + 1):
+        if window_ok(k, bits) and abs(k - ct) < abs(best - ct):
+            best = k
+    return best
 
 
 class HeuristicPlanner:
@@ -129,14 +151,26 @@ class MlpPlanner:
     def plan(self, curve: int, sizes, batch: int):
         sizes = np.asarray(sizes)
         x = features(np.log2(np.maximum(sizes, 1)), np.full(len(sizes), batch), np.full(len(sizes), curve - 1))
-        dec = decode(self.predict(x))
+        return self.plans_from_outputs(curve, sizes, self.predict(x))
+
+    def plans_from_outputs(self, curve: int, sizes, y: np.ndarray):
+        """Network outputs (one row per MSM) -> (windows, segs); a row that
+        decodes to nothing usable leaves that MSM to the heuristic (0, 0)."""
+        import gpu_msm_amd as gm
+
+        dec = decode(y)
         w = np.zeros(len(sizes), dtype=np.int32)
         s = np.zeros(len(sizes), dtype=np.int32)
+        heur = {}
         for k, d in enumerate(dec):
             if d is None:
                 self.fallbacks += 1  # heuristic for this MSM
                 continue
-            w[k], s[k] = d
+            n = int(sizes[k])
+            if n not in heur:
+                heur[n] = gm.plan_query(curve, max(n, 1), 5, 299)[0]
+            c = snap_window(d[0], heur[n])
+            w[k], s[k] = c, min(d[1], 1 << (c - 1))
         return w, s
 
 
