@@ -11,6 +11,9 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <atomic>
+#include <memory>
+#include <mutex>
 #include <vector>
 
 #include "../../include/curve.h"
@@ -83,6 +86,55 @@ void mont_batch(hipStream_t s, uint32_t gpu, P* pts, int n, bool sync, const cha
 
 // --------------------------------------------------------------- MSM
 
+// Stream-ordered host finish of the asynchronous MSM. The window combine is a
+// serial chain of about (scalar bits) doublings: one GPU lane runs each
+// dependent 446-bit product at a few microseconds, a host core at a tenth of
+// that, so the asynchronous entry points finish on the host without blocking
+// the caller: the window sums are copied into a pinned slot, a host function
+// enqueued on the caller's stream combines them and writes the ABI record into
+// the slot, the record is copied to the caller's device pointer, and a last
+// host function frees the slot. Every step is ordered on the stream; slots are
+// reused, never freed before the process ends.
+struct FinishSlot {
+    uint8_t* buf = nullptr;  // pinned: window sums, then the record at res_off
+    size_t cap = 0;
+    std::atomic<bool> busy{false};
+    int nw = 0, c = 0;
+    bool proj = false;
+    size_t res_off = 0;
+};
+
+std::mutex g_slot_mu;
+std::vector<std::unique_ptr<FinishSlot>> g_slots;
+
+FinishSlot* acquire_slot(size_t bytes, const char* fn) {
+    std::lock_guard<std::mutex> lk(g_slot_mu);
+    for (auto& sl : g_slots) {
+        bool idle = false;
+        if (sl->cap >= bytes && sl->busy.compare_exchange_strong(idle, true)) return sl.get();
+    }
+    auto sl = std::make_unique<FinishSlot>();
+    check(hipHostMalloc(reinterpret_cast<void**>(&sl->buf), bytes, hipHostMallocPortable), fn);
+    sl->cap = bytes;
+    sl->busy = true;
+    g_slots.push_back(std::move(sl));
+    return g_slots.back().get();
+}
+
+template <class O>
+void finish_slot(void* arg) {
+    using HF = typename gmc::HostOf<O>::HF;
+    FinishSlot* sl = static_cast<FinishSlot*>(arg);
+    HF x, y;
+    const bool finite = gmc::host_combine<HF>(x, y, reinterpret_cast<const uint32_t*>(sl->buf), sl->nw, sl->c);
+    if (sl->proj)
+        gmc::write_proj_abi<HF>(sl->buf + sl->res_off, finite, x, y);
+    else
+        gmc::write_affine_abi<HF>(sl->buf + sl->res_off, finite, x, y);
+}
+
+void release_slot(void* arg) { static_cast<FinishSlot*>(arg)->busy.store(false); }
+
 template <class O>
 bool plan_one(Plan& p, gm::MsmDesc& d, int n, int limbs) {
     gm::MsmSpec spec = {n, 0, 0, 0, 0};
@@ -132,14 +184,20 @@ void msm_device(hipStream_t s, uint32_t gpu, void* d_result, const void* d_point
     check(gm::Engine<O>::import_aff(s, d_aff, d_points, n, false), fn);
     check(gm::Engine<O>::run(s, p, d_aff, d_scalars, ws, nullptr), fn);
     if (!sync) {
-        // device window combine, then the record, all stream-ordered
-        check(gm::Engine<O>::device_combine(s, p, ws), fn);
-        const Aff<O>* out = reinterpret_cast<const Aff<O>*>(ws + p.o_out);
-        if (proj_out)
-            check(gm::launch_aff_to_abi_proj<O>(s, d_result, out, 1), fn);
-        else
-            check(gm::launch_export<O>(s, d_result, out, 1), fn);
+        // stream-ordered host finish (FinishSlot): nothing here waits
+        const size_t wb = (size_t)d.nw * sizeof(Xyzz<O>), res_off = gm::al_size(wb);
+        FinishSlot* sl = acquire_slot(res_off + Lay::PROJ_BYTES, fn);
+        sl->nw = (int)d.nw;
+        sl->c = (int)d.c;
+        sl->proj = proj_out;
+        sl->res_off = res_off;
+        check(hipMemcpyAsync(sl->buf, ws + p.o_wsum, wb, hipMemcpyDeviceToHost, s), fn);
         if (own) check(hipFreeAsync(scratch, s), fn);
+        check(hipLaunchHostFunc(s, finish_slot<O>, sl), fn);
+        check(hipMemcpyAsync(d_result, sl->buf + res_off, proj_out ? Lay::PROJ_BYTES : Lay::AFF_BYTES,
+                             hipMemcpyHostToDevice, s),
+              fn);
+        check(hipLaunchHostFunc(s, release_slot, sl), fn);
         return;
     }
     // synchronous: window sums to the host, Horner there, the record back

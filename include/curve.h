@@ -102,10 +102,15 @@ void point_to_montgomery_batch(hipStream_t stream, uint32_t gpu_index, PointType
  * A caller that cannot size it passes d_scratch = NULL: the call then takes
  * and returns the memory stream-ordered (hipMallocAsync / hipFreeAsync).
  *
- * The _async forms finish entirely on the device (window combine included)
- * and never block the host; the synchronous forms finish the window combine
- * on the host, which is faster for a single MSM. Both give identical results.
- * Calls on different streams may overlap when their scratch buffers differ. */
+ * The _async forms never block the host: the window sums go to pinned host
+ * memory, a host function enqueued on the stream (hipLaunchHostFunc) runs the
+ * window combine, and the record is copied to d_result, all in stream order
+ * (a serial chain of ~bits doublings runs about ten times faster on a host
+ * core than on one GPU lane). The synchronous forms do the same combine
+ * inline and return when d_result is written. Both give identical results.
+ * Calls on different streams may overlap when their scratch buffers differ.
+ * The _async forms are not meant for hipGraph capture (their pinned slot is
+ * taken when the call is made). */
 size_t point_msm_u64_scratch_bytes_g1(int n);
 size_t point_msm_u64_scratch_bytes_g2(int n);
 template <typename PointType>

@@ -250,6 +250,32 @@ def test_point_msm_u64_device_api_vs_oracle(S, curve, n):
         assert not to_host(d_out, np.uint8).any(), "n = 0 must give the all-zero (Z = 0) point"
 
 
+@pytest.mark.parametrize("curve", ["g1", "g2"])
+def test_point_msm_u64_async_back_to_back(S, curve):
+    """Six point_msm_u64_async calls enqueued on one stream with no host wait
+    between them (one shared scratch buffer, then NULL scratch), each into its
+    own result record: the stream-ordered host finishes (pinned slots taken and
+    released in stream order) must give every MSM its own oracle result."""
+    sizes = [700, 1, 64, 513, 2, 1000]
+    ks = o.gen_u64(930, max(sizes))
+    bases = o.bases_from_k(curve, ks)
+    sc = o.gen_u64(931, max(sizes))
+    q = S.fn(f"point_msm_u64_scratch_bytes_{curve}", [I32], ctypes.c_size_t)
+    scratch = empty_dev(q(max(sizes)))
+    d_b, d_s = to_dev(bases), to_dev(sc)
+    proj_dt = o.G1P_DT if curve == "g1" else o.G2P_DT
+    f = S.fn(f"point_msm_u64_async_{curve}", [P, U32, P, P, P, P, I32])
+    for use_scratch in (True, False):
+        d_out = empty_dev(len(sizes) * proj_dt.itemsize)
+        for k, n in enumerate(sizes):
+            f(stream(), 0, d_out.data_ptr() + k * proj_dt.itemsize, d_b.data_ptr(), d_s.data_ptr(),
+              scratch.data_ptr() if use_scratch else None, n)
+        got = to_host(d_out, proj_dt)
+        for k, n in enumerate(sizes):
+            want = o.msm(curve, bases[:n], sc[:n].reshape(-1, 1))
+            assert fields(o.to_affine(curve, got[k:k + 1])) == fields(want), (use_scratch, k, n)
+
+
 @pytest.mark.parametrize("curve,n", [("g1", 2000), ("g2", 300)])
 def test_point_msm_multi_limb_device_api_vs_oracle(S, curve, n):
     """point_msm<PointType> / point_msm_async (reference
