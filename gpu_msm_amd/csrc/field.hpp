@@ -601,10 +601,20 @@ struct Fp2LaneOps {
         fp_pair_swap(bo, b);
         fp2_lane_mul(r, a, ao, b, bo);
     }
+    // (a0 + a1 i)^2 = (a0 + a1)(a0 - a1) + 2 a0 a1 i: one plain product per
+    // lane (part 0: (a0 + a1)(a0 - a1), part 1: (a1 + a1) a0), operands
+    // picked per lane by selects so both lanes run one instruction stream.
     __device__ static __forceinline__ void sqr(F& r, const F& a) {
-        Fp ao;
+        Fp ao, x, y, t;
         fp_pair_swap(ao, a);
-        fp2_lane_mul(r, a, ao, a, ao);
+        const bool q = lane_part() != 0;
+#pragma unroll
+        for (int i = 0; i < NL; ++i) t.v[i] = q ? a.v[i] : ao.v[i];
+        fp_add(x, a, t);
+        fp_sub(y, a, ao);
+#pragma unroll
+        for (int i = 0; i < NL; ++i) y.v[i] = q ? ao.v[i] : y.v[i];
+        fp_mul(r, x, y);
     }
     __device__ static __forceinline__ void mul_sub(F& r, const F& a, const F& b, const F& c, const F& d) {
         Fp ao, bo, co, dx;

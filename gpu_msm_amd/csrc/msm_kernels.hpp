@@ -3,24 +3,22 @@
 //
 // Pipeline for one MSM over N points with c-bit signed windows,
 // nw windows and B = 2^(c-1) buckets per window:
-//   k_digits_hist  signed-digit recoding of each scalar (reference
-//                  src/curve.cu:892-922 extracts unsigned windows; here digits
-//                  are in [-2^(c-1), 2^(c-1)] and the sign negates the base's
-//                  y), counting each (window, bucket) with an atomic whose
-//                  return value is the entry's rank inside its bucket
-//   scan           exclusive prefix sum of the counts (hipcub)
-//   k_scatter      writes point index | sign bit at offset[bucket] + rank,
-//                  so each bucket's points are contiguous
-//   k_accumulate   one lane per bucket: gathers its affine bases (128 B G1 /
-//                  256 B G2 records, one or two cache lines) and mixed-adds
-//                  them into an XYZZ accumulator
+//   k_digits       signed-digit recoding of each scalar into 16-bit codes
+//                  (reference src/curve.cu:892-922 extracts unsigned windows;
+//                  here digits are in [-2^(c-1), 2^(c-1)] and the sign negates
+//                  the base's y)
+//   k_tile_count / scan / k_tile_place  counting sort of the (window, point)
+//                  entries by bucket, tiles counted in LDS (sort_kernels.hpp)
+//   k_accumulate   each lane mixed-adds a fixed chunk of sorted entries into
+//                  XYZZ accumulators; runs cut by a chunk edge go to head/tail
+//                  partials that k_bucket_fixup sums
 //   k_bucket_reduce one lane per segment of buckets: running sums give
 //                  sum_b (b+1) B_b for the segment as T + lo * S
 //   k_window_partial / k_window_sum  tiles of segments, then each window's
 //                  total over its tiles (two short tree stages)
 //   combine        Horner over windows, most significant first (reference
 //                  src/curve.cu:2060-2179 combines on the CPU after every
-//                  window; here either on the host once, or on the device)
+//                  window; here on the host once, or k_horner on the device)
 #pragma once
 #include "curve.hpp"
 #include "desc_find.hpp"
