@@ -252,8 +252,25 @@ __global__ void __launch_bounds__(64) k_accumulate(const MsmDesc* __restrict__ d
     xyzz_set_inf(acc);
     uint32_t cur = (uint32_t)(entries[j0] >> 32);
     bool first = true;
+    // G1 (one lane per item, 202 registers) loads the next entry's base while
+    // the current mixed add runs; a G2 lane pair has no registers to spare.
+    constexpr bool PREFETCH = Stor<O>::LPI == 1;
+    uint64_t en = entries[j0];
+    Aff<O> nx;
+    if constexpr (PREFETCH) ldp(nx, &bases[(uint32_t)en & 0x7FFFFFFFu]);
     for (uint32_t j = j0; j < j1; ++j) {
-        const uint64_t e = entries[j];
+        const uint64_t e = en;
+        Aff<O> pt;
+        if constexpr (PREFETCH) {
+            pt = nx;
+            if (j + 1 < j1) {
+                en = entries[j + 1];
+                ldp(nx, &bases[(uint32_t)en & 0x7FFFFFFFu]);
+            }
+        } else {
+            ldp(pt, &bases[(uint32_t)e & 0x7FFFFFFFu]);
+            if (j + 1 < j1) en = entries[j + 1];
+        }
         const uint32_t g = (uint32_t)(e >> 32);
         if (g != cur) {
             if (first && off[cur] < j0)
@@ -264,8 +281,6 @@ __global__ void __launch_bounds__(64) k_accumulate(const MsmDesc* __restrict__ d
             cur = g;
             first = false;
         }
-        Aff<O> pt;
-        ldp(pt, &bases[(uint32_t)e & 0x7FFFFFFFu]);
         if (aff_is_inf(pt)) continue;
         if ((uint32_t)e >> 31) O::neg(pt.y, pt.y);
         xyzz_add_aff<O>(acc, pt.x, pt.y);
