@@ -3,7 +3,10 @@
 // Bucket accumulators use extended Jacobian "XYZZ" coordinates
 // (x = X/ZZ, y = Y/ZZZ, ZZ^3 = ZZZ^2), whose mixed addition costs 8M + 2S and
 // needs no curve constant. Each formula's final Y3 = A B - C D is one lazily
-// reduced difference of products (O::mul_sub). The point at infinity is ZZ = ZZZ = 0. Results are
+// reduced difference of products (O::mul_sub_raw), and differences that only
+// feed products stay raw (O::sub_raw: no carry normalisation; for a field
+// policy without a raw form these are the normalised operations). The point
+// at infinity is ZZ = ZZZ = 0. Results are
 // compared only after affine normalisation, where every representation of a
 // point gives the same canonical (x, y) as the reference's standard-projective
 // formulas (reference src/curve.cu:444-778).
@@ -72,8 +75,8 @@ __device__ __forceinline__ void xyzz_dbl_aff(Xyzz<O>& r, const typename O::F& x,
     O::sqr(t, M);
     O::sub(t, t, S);
     O::sub(r.X, t, S);   // X3 = M^2 - 2S
-    O::sub(t, S, r.X);
-    O::mul_sub(r.Y, M, t, W, y);  // Y3 = M(S - X3) - W y
+    O::sub_raw(t, S, r.X);
+    O::mul_sub_raw(r.Y, M, t, W, y);  // Y3 = M(S - X3) - W y
     r.ZZ = V;
     r.ZZZ = W;
 }
@@ -99,8 +102,8 @@ __device__ __forceinline__ void xyzz_dbl(Xyzz<O>& r, const Xyzz<O>& p) {
     O::sqr(t, M);
     O::sub(t, t, S);
     O::sub(r.X, t, S);
-    O::sub(t, S, r.X);
-    O::mul_sub(r.Y, M, t, W, p.Y);  // p.Y is read before r.Y is written (r may alias p)
+    O::sub_raw(t, S, r.X);
+    O::mul_sub_raw(r.Y, M, t, W, p.Y);  // p.Y is read before r.Y is written (r may alias p)
 }
 
 // acc += (x2, y2) affine (madd-2008-s). Handles acc = O, equal points
@@ -117,28 +120,29 @@ __device__ __forceinline__ void xyzz_add_aff(Xyzz<O>& acc, const typename O::F& 
     }
     F P, R, PP, PPP, Q, t;
     O::mul(P, x2, acc.ZZ);
-    O::sub(P, P, acc.X);     // P = U2 - X1
+    O::sub_raw(P, P, acc.X); // P = U2 - X1 (raw: a product operand only)
     O::mul(R, y2, acc.ZZZ);
-    O::sub(R, R, acc.Y);     // R = S2 - Y1
-    if (O::is_zero(P)) {
-        if (O::is_zero(R)) {
+    O::sub_raw(R, R, acc.Y); // R = S2 - Y1 (raw)
+    O::sqr_raw(PP, P);
+    if (O::is_zero(PP)) {    // P = 0 (mod p): the same x
+        O::sqr_raw(t, R);
+        if (O::is_zero(t)) {
             xyzz_dbl_aff<O>(acc, x2, y2);
         } else {
             xyzz_set_inf(acc);
         }
         return;
     }
-    O::sqr(PP, P);
-    O::mul(PPP, P, PP);
+    O::mul_raw(PPP, PP, P);
     O::mul(Q, acc.X, PP);
     O::mul(acc.ZZ, acc.ZZ, PP);
     O::mul(acc.ZZZ, acc.ZZZ, PPP);
-    O::sqr(t, R);
+    O::sqr_raw(t, R);
     O::sub(t, t, PPP);
     O::sub(t, t, Q);
     O::sub(acc.X, t, Q);     // X3 = R^2 - PPP - 2Q
-    O::sub(t, Q, acc.X);
-    O::mul_sub(acc.Y, R, t, acc.Y, PPP);  // Y3 = R(Q - X3) - Y1 PPP, one reduction
+    O::sub_raw(t, Q, acc.X);
+    O::mul_sub_raw(acc.Y, R, t, acc.Y, PPP);  // Y3 = R(Q - X3) - Y1 PPP, one reduction
 }
 
 // r = p + q, both XYZZ (add-2008-s). r may alias p or q.
@@ -156,31 +160,32 @@ __device__ __forceinline__ void xyzz_add(Xyzz<O>& r, const Xyzz<O>& p, const Xyz
     F U1, S1, P, R, PP, PPP, Q, t;
     O::mul(U1, p.X, q.ZZ);
     O::mul(P, q.X, p.ZZ);
-    O::sub(P, P, U1);        // P = U2 - U1
+    O::sub_raw(P, P, U1);    // P = U2 - U1 (raw: a product operand only)
     O::mul(S1, p.Y, q.ZZZ);
     O::mul(R, q.Y, p.ZZZ);
-    O::sub(R, R, S1);        // R = S2 - S1
-    if (O::is_zero(P)) {
-        if (O::is_zero(R)) {
+    O::sub_raw(R, R, S1);    // R = S2 - S1 (raw)
+    O::sqr_raw(PP, P);
+    if (O::is_zero(PP)) {    // P = 0 (mod p): the same x
+        O::sqr_raw(t, R);
+        if (O::is_zero(t)) {
             xyzz_dbl<O>(r, p);
         } else {
             xyzz_set_inf(r);
         }
         return;
     }
-    O::sqr(PP, P);
-    O::mul(PPP, P, PP);
+    O::mul_raw(PPP, PP, P);
     O::mul(Q, U1, PP);
     O::mul(t, p.ZZ, q.ZZ);
     O::mul(r.ZZ, t, PP);
     O::mul(t, p.ZZZ, q.ZZZ);
     O::mul(r.ZZZ, t, PPP);
-    O::sqr(t, R);
+    O::sqr_raw(t, R);
     O::sub(t, t, PPP);
     O::sub(t, t, Q);
     O::sub(r.X, t, Q);
-    O::sub(t, Q, r.X);
-    O::mul_sub(r.Y, R, t, S1, PPP);
+    O::sub_raw(t, Q, r.X);
+    O::mul_sub_raw(r.Y, R, t, S1, PPP);
 }
 
 // r = -p

@@ -337,6 +337,81 @@ __host__ __device__ __forceinline__ void fp_mul_sub(Fp& r, const Fp& a, const Fp
     redc_fold(r, t, top, false);
 }
 
+// Raw differences. A value used only as a product operand can skip the
+// carry normalisation and the 2p correction of fp_sub: r = a - b limb by limb
+// (signed 32-bit limbs, |limb| < 2^28, value in (-2p, 2p) for a, b in
+// [0, 2p)). The signed products below accept such operands: every limb
+// product is a signed 32 x 32 -> 64 mad, and a product of two values below 2p
+// in magnitude is below 4p^2, the same bound as for [0, 2p) operands.
+__device__ __forceinline__ void fp_sub_raw(Fp& r, const Fp& a, const Fp& b) {
+#pragma unroll
+    for (int i = 0; i < NL; ++i) r.v[i] = a.v[i] - b.v[i];
+}
+
+// r = a^2 R^{-1} for a raw a: the square is non-negative and below 4p^2, so
+// the reduced value is in [0, 2p) with no correction (off-diagonal products
+// once against doubled limbs, |2 a_i| < 2^29).
+__device__ __forceinline__ void fp_sqr_raw(Fp& r, const Fp& a) {
+    uint32_t m[NL];
+    int32_t d[NL], s[NL];
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+        s[i] = (int32_t)a.v[i];
+        d[i] = s[i] * 2;
+    }
+    int64_t acc = 0;
+#pragma unroll
+    for (int k = 0; k < NL; ++k) {
+#pragma unroll
+        for (int i = 0; i < k - i; ++i) acc += (int64_t)d[i] * s[k - i];
+        if ((k & 1) == 0) acc += (int64_t)s[k >> 1] * s[k >> 1];
+#pragma unroll
+        for (int i = 0; i < k; ++i) acc += (int64_t)((uint64_t)m[i] * PL[k - i]);
+        m[k] = ((uint32_t)acc * PINV28) & MASK28;
+        acc += (int64_t)((uint64_t)m[k] * PL[0]);
+        acc >>= 28;  // arithmetic: the low 28 bits are zero
+    }
+#pragma unroll
+    for (int k = NL; k < 2 * NL - 1; ++k) {
+#pragma unroll
+        for (int i = k - NL + 1; i < k - i; ++i) acc += (int64_t)d[i] * s[k - i];
+        if ((k & 1) == 0) acc += (int64_t)s[k >> 1] * s[k >> 1];
+#pragma unroll
+        for (int i = k - NL + 1; i < NL; ++i) acc += (int64_t)((uint64_t)m[i] * PL[k - i]);
+        r.v[k - NL] = (uint32_t)acc & MASK28;
+        acc >>= 28;
+    }
+    r.v[NL - 1] = (uint32_t)acc;  // the value is in [0, 2p): the top limb is non-negative
+}
+
+// r = a b R^{-1} for a in [0, 2p) and a raw b: the product lies in
+// (-4p^2, 4p^2), the reduced value in (-p, 2p), and one conditional addition
+// of 2p lands in [0, 2p).
+__device__ __forceinline__ void fp_mul_raw(Fp& r, const Fp& a, const Fp& b) {
+    const Fp* const x[1] = {&a};
+    const int32_t(*const y[1])[NL] = {reinterpret_cast<const int32_t(*)[NL]>(&b.v)};
+    uint32_t t[NL];
+    int64_t top;
+    redc_signed<1>(t, top, x, y);
+    redc_fold(r, t, top, false);
+}
+
+// r = (a b - c d) R^{-1} for raw a, b and c, d in [0, 2p): a b lies in
+// (-4p^2, 4p^2) and c d in [0, 4p^2), so the difference lies in (-8p^2, 4p^2)
+// and the reduced value in (-1.9p, 1.95p) (8p^2 / R < 1.9p as p < 0.95 R / 4);
+// one conditional addition of 2p lands in [0, 2p).
+__device__ __forceinline__ void fp_mul_sub_raw(Fp& r, const Fp& a, const Fp& b, const Fp& c, const Fp& d) {
+    int32_t nd[NL];
+#pragma unroll
+    for (int i = 0; i < NL; ++i) nd[i] = -(int32_t)d.v[i];
+    const Fp* const x[2] = {&a, &c};
+    const int32_t(*const y[2])[NL] = {reinterpret_cast<const int32_t(*)[NL]>(&b.v), &nd};
+    uint32_t t[NL];
+    int64_t top;
+    redc_signed<2>(t, top, x, y);
+    redc_fold(r, t, top, false);
+}
+
 // ------------------------------------------------------------------ Fp2
 // Fp2 = Fp[i]/(i^2 + 1) (non-residue -1 as in reference include/fp2.h:7).
 
@@ -480,6 +555,15 @@ struct FpOps {
     __device__ static __forceinline__ void mul_sub(F& r, const F& a, const F& b, const F& c, const F& d) {
         fp_mul_sub(r, a, b, c, d);
     }
+    // Raw differences (operands of products only) and the products taking
+    // them: sqr_raw(raw), mul_raw(normalised, raw), mul_sub_raw(raw, raw,
+    // normalised, normalised); all results are normalised to [0, 2p).
+    __device__ static __forceinline__ void sub_raw(F& r, const F& a, const F& b) { fp_sub_raw(r, a, b); }
+    __device__ static __forceinline__ void sqr_raw(F& r, const F& a) { fp_sqr_raw(r, a); }
+    __device__ static __forceinline__ void mul_raw(F& r, const F& a, const F& b) { fp_mul_raw(r, a, b); }
+    __device__ static __forceinline__ void mul_sub_raw(F& r, const F& a, const F& b, const F& c, const F& d) {
+        fp_mul_sub_raw(r, a, b, c, d);
+    }
     __device__ static __forceinline__ bool is_zero(const F& a) { return fp_is_zero(a); }
     __device__ static __forceinline__ bool is_raw_zero(const F& a) { return fp_is_raw_zero(a); }
     __device__ static __forceinline__ void canon(F& r, const F& a) { fp_canon(r, a); }
@@ -502,6 +586,13 @@ struct Fp2Ops {
         fp2_mul(t, c, d);
         fp2_mul(r, a, b);
         fp2_sub(r, r, t);
+    }
+    // no raw form: the "raw" operations are the normalised ones
+    __device__ static __forceinline__ void sub_raw(F& r, const F& a, const F& b) { sub(r, a, b); }
+    __device__ static __forceinline__ void sqr_raw(F& r, const F& a) { sqr(r, a); }
+    __device__ static __forceinline__ void mul_raw(F& r, const F& a, const F& b) { mul(r, a, b); }
+    __device__ static __forceinline__ void mul_sub_raw(F& r, const F& a, const F& b, const F& c, const F& d) {
+        mul_sub(r, a, b, c, d);
     }
     __device__ static __forceinline__ bool is_zero(const F& a) { return fp2_is_zero(a); }
     __device__ static __forceinline__ bool is_raw_zero(const F& a) { return fp2_is_raw_zero(a); }
@@ -623,6 +714,13 @@ struct Fp2LaneOps {
         fp_pair_swap(co, c);
         fp_pair_swap(dx, d);
         fp2_lane_mul_sub(r, a, ao, b, bo, c, co, d, dx);
+    }
+    // no raw form: the "raw" operations are the normalised ones
+    __device__ static __forceinline__ void sub_raw(F& r, const F& a, const F& b) { sub(r, a, b); }
+    __device__ static __forceinline__ void sqr_raw(F& r, const F& a) { sqr(r, a); }
+    __device__ static __forceinline__ void mul_raw(F& r, const F& a, const F& b) { mul(r, a, b); }
+    __device__ static __forceinline__ void mul_sub_raw(F& r, const F& a, const F& b, const F& c, const F& d) {
+        mul_sub(r, a, b, c, d);
     }
     __device__ static __forceinline__ bool is_zero(const F& a) {
         const uint32_t z = fp_is_zero(a) ? 1u : 0u;

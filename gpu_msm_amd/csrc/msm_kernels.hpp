@@ -236,7 +236,7 @@ __global__ void __launch_bounds__(64) k_gen_bases(Aff<O>* __restrict__ out, cons
 // head[lane], one cut only by its end to tail[lane], and k_bucket_fixup sums
 // those partials.
 template <class O>
-__global__ void __launch_bounds__(64) k_accumulate(const MsmDesc* __restrict__ desc, int count, uint32_t lanes,
+__global__ void __launch_bounds__(64, 2) k_accumulate(const MsmDesc* __restrict__ desc, int count, uint32_t lanes,
                                                    const typename Stor<O>::A* __restrict__ bases,
                                                    const uint64_t* __restrict__ entries,
                                                    const uint32_t* __restrict__ off, typename Stor<O>::X* __restrict__ buckets,
