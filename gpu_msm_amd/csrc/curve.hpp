@@ -60,6 +60,33 @@ __device__ __forceinline__ bool xyzz_is_inf(const Xyzz<O>& p) {
     return O::is_zero(p.ZZ);
 }
 
+// X3 = a^2 - b - 2c (a raw) and X3 = a^2 - 2c: one fused reduction where the
+// field policy has it (O::RAW), else a square and normalised subtractions
+// through the scratch t.
+template <class O>
+__device__ __forceinline__ void x3_sqr_sub2(typename O::F& r, const typename O::F& a, const typename O::F& b,
+                                            const typename O::F& c, typename O::F& t) {
+    if constexpr (O::RAW) {
+        O::sqr_sub2(r, a, b, c);
+    } else {
+        O::sqr(t, a);
+        O::sub(t, t, b);
+        O::sub(t, t, c);
+        O::sub(r, t, c);
+    }
+}
+template <class O>
+__device__ __forceinline__ void x3_sqr_sub2z(typename O::F& r, const typename O::F& a, const typename O::F& c,
+                                             typename O::F& t) {
+    if constexpr (O::RAW) {
+        O::sqr_sub2z(r, a, c);
+    } else {
+        O::sqr(t, a);
+        O::sub(t, t, c);
+        O::sub(r, t, c);
+    }
+}
+
 // 2 * (x, y) for affine input, result in XYZZ (mdbl-2008-s-1).
 template <class O>
 __device__ __forceinline__ void xyzz_dbl_aff(Xyzz<O>& r, const typename O::F& x, const typename O::F& y) {
@@ -72,9 +99,7 @@ __device__ __forceinline__ void xyzz_dbl_aff(Xyzz<O>& r, const typename O::F& x,
     O::sqr(t, x);        // x^2
     O::dbl(M, t);
     O::add(M, M, t);     // M = 3 x^2
-    O::sqr(t, M);
-    O::sub(t, t, S);
-    O::sub(r.X, t, S);   // X3 = M^2 - 2S
+    x3_sqr_sub2z<O>(r.X, M, S, t);  // X3 = M^2 - 2S
     O::sub_raw(t, S, r.X);
     O::mul_sub_raw(r.Y, M, t, W, y);  // Y3 = M(S - X3) - W y
     r.ZZ = V;
@@ -99,9 +124,7 @@ __device__ __forceinline__ void xyzz_dbl(Xyzz<O>& r, const Xyzz<O>& p) {
     O::add(M, M, t);
     O::mul(r.ZZ, V, p.ZZ);
     O::mul(r.ZZZ, W, p.ZZZ);
-    O::sqr(t, M);
-    O::sub(t, t, S);
-    O::sub(r.X, t, S);
+    x3_sqr_sub2z<O>(r.X, M, S, t);  // X3 = M^2 - 2S
     O::sub_raw(t, S, r.X);
     O::mul_sub_raw(r.Y, M, t, W, p.Y);  // p.Y is read before r.Y is written (r may alias p)
 }
@@ -137,10 +160,7 @@ __device__ __forceinline__ void xyzz_add_aff(Xyzz<O>& acc, const typename O::F& 
     O::mul(Q, acc.X, PP);
     O::mul(acc.ZZ, acc.ZZ, PP);
     O::mul(acc.ZZZ, acc.ZZZ, PPP);
-    O::sqr_raw(t, R);
-    O::sub(t, t, PPP);
-    O::sub(t, t, Q);
-    O::sub(acc.X, t, Q);     // X3 = R^2 - PPP - 2Q
+    x3_sqr_sub2<O>(acc.X, R, PPP, Q, t);  // X3 = R^2 - PPP - 2Q
     O::sub_raw(t, Q, acc.X);
     O::mul_sub_raw(acc.Y, R, t, acc.Y, PPP);  // Y3 = R(Q - X3) - Y1 PPP, one reduction
 }
@@ -180,10 +200,7 @@ __device__ __forceinline__ void xyzz_add(Xyzz<O>& r, const Xyzz<O>& p, const Xyz
     O::mul(r.ZZ, t, PP);
     O::mul(t, p.ZZZ, q.ZZZ);
     O::mul(r.ZZZ, t, PPP);
-    O::sqr_raw(t, R);
-    O::sub(t, t, PPP);
-    O::sub(t, t, Q);
-    O::sub(r.X, t, Q);
+    x3_sqr_sub2<O>(r.X, R, PPP, Q, t);  // X3 = R^2 - PPP - 2Q
     O::sub_raw(t, Q, r.X);
     O::mul_sub_raw(r.Y, R, t, S1, PPP);
 }

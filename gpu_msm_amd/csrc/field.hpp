@@ -384,6 +384,73 @@ __device__ __forceinline__ void fp_sqr_raw(Fp& r, const Fp& a) {
     r.v[NL - 1] = (uint32_t)acc;  // the value is in [0, 2p): the top limb is non-negative
 }
 
+// 4p in 28-bit limbs (4p < R, so 16 limbs hold it)
+constexpr uint32_t P4L[NL] = {0xac2aaac, 0x70009aa, 0xed730c4, 0x14a315c, 0x7b95bb9, 0x0f88eb3,
+                              0xa0939b7, 0xeb1c580, 0xdd6992e, 0x0bf8c83, 0x692de28, 0x5900025,
+                              0xd2596ab, 0xd4803f0, 0xca31794, 0xf37b83e};
+
+// Subtracts m (4p or 2p limbs) from the normalised value t when that does not
+// borrow.
+__device__ __forceinline__ void fp_cond_sub(uint32_t (&t)[NL], const uint32_t (&m)[NL]) {
+    uint32_t d[NL];
+    int32_t br = 0;
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+        const int32_t v = (int32_t)t[i] - (int32_t)m[i] + br;
+        d[i] = (uint32_t)v & MASK28;
+        br = v >> 28;
+    }
+    const bool keep = br < 0;
+#pragma unroll
+    for (int i = 0; i < NL; ++i) t[i] = keep ? t[i] : d[i];
+}
+
+// r = a^2 R^{-1} - b - 2c for a raw a and b, c in [0, 2p), in [0, 2p) (the
+// X3 of every XYZZ formula). b + 2c - 6p is subtracted from the high half of
+// the square's column sums, i.e. (b + 2c - 6p) R before the reduction: the
+// square reduces to [0, 1.95p) and b + 2c < 6p, so the result lies in
+// (0, 8p) with normalised limbs, and two conditional subtractions (4p, then
+// 2p) bring it to [0, 2p). Limb i of 6p - b - 2c is 6 p_i - b_i - 2 c_i, in
+// (-3 * 2^28, 2^31).
+__device__ __forceinline__ void fp_sqr_raw_sub2(Fp& r, const Fp& a, const Fp& b, const Fp& c) {
+    uint32_t m[NL], t[NL];
+    int32_t d[NL], s[NL];
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+        s[i] = (int32_t)a.v[i];
+        d[i] = s[i] * 2;
+    }
+    int64_t acc = 0;
+#pragma unroll
+    for (int k = 0; k < NL; ++k) {
+#pragma unroll
+        for (int i = 0; i < k - i; ++i) acc += (int64_t)d[i] * s[k - i];
+        if ((k & 1) == 0) acc += (int64_t)s[k >> 1] * s[k >> 1];
+#pragma unroll
+        for (int i = 0; i < k; ++i) acc += (int64_t)((uint64_t)m[i] * PL[k - i]);
+        m[k] = ((uint32_t)acc * PINV28) & MASK28;
+        acc += (int64_t)((uint64_t)m[k] * PL[0]);
+        acc >>= 28;
+    }
+#pragma unroll
+    for (int k = NL; k < 2 * NL - 1; ++k) {
+#pragma unroll
+        for (int i = k - NL + 1; i < k - i; ++i) acc += (int64_t)d[i] * s[k - i];
+        if ((k & 1) == 0) acc += (int64_t)s[k >> 1] * s[k >> 1];
+#pragma unroll
+        for (int i = k - NL + 1; i < NL; ++i) acc += (int64_t)((uint64_t)m[i] * PL[k - i]);
+        acc += (int64_t)((int32_t)(6u * PL[k - NL]) - (int32_t)b.v[k - NL] - 2 * (int32_t)c.v[k - NL]);
+        t[k - NL] = (uint32_t)acc & MASK28;
+        acc >>= 28;
+    }
+    acc += (int64_t)((int32_t)(6u * PL[NL - 1]) - (int32_t)b.v[NL - 1] - 2 * (int32_t)c.v[NL - 1]);
+    t[NL - 1] = (uint32_t)acc;  // in (0, 8p): non-negative, below 2^29
+    fp_cond_sub(t, P4L);
+    fp_cond_sub(t, P2L);
+#pragma unroll
+    for (int i = 0; i < NL; ++i) r.v[i] = t[i];
+}
+
 // r = a b R^{-1} for a in [0, 2p) and a raw b: the product lies in
 // (-4p^2, 4p^2), the reduced value in (-p, 2p), and one conditional addition
 // of 2p lands in [0, 2p).
@@ -545,6 +612,7 @@ __device__ __forceinline__ void fp2_inv(Fp2& r, const Fp2& a) {
 struct FpOps {
     using F = Fp;
     static constexpr int WORDS = NL;
+    static constexpr bool RAW = true;  // has sqr_sub2 / sqr_sub2z
     __device__ static __forceinline__ void add(F& r, const F& a, const F& b) { fp_add(r, a, b); }
     __device__ static __forceinline__ void sub(F& r, const F& a, const F& b) { fp_sub(r, a, b); }
     __device__ static __forceinline__ void dbl(F& r, const F& a) { fp_dbl(r, a); }
@@ -564,6 +632,15 @@ struct FpOps {
     __device__ static __forceinline__ void mul_sub_raw(F& r, const F& a, const F& b, const F& c, const F& d) {
         fp_mul_sub_raw(r, a, b, c, d);
     }
+    // r = a^2 - b - 2c and r = a^2 - 2c (a raw)
+    __device__ static __forceinline__ void sqr_sub2(F& r, const F& a, const F& b, const F& c) {
+        fp_sqr_raw_sub2(r, a, b, c);
+    }
+    __device__ static __forceinline__ void sqr_sub2z(F& r, const F& a, const F& c) {
+        Fp z;
+        fp_set_zero(z);
+        fp_sqr_raw_sub2(r, a, z, c);
+    }
     __device__ static __forceinline__ bool is_zero(const F& a) { return fp_is_zero(a); }
     __device__ static __forceinline__ bool is_raw_zero(const F& a) { return fp_is_raw_zero(a); }
     __device__ static __forceinline__ void canon(F& r, const F& a) { fp_canon(r, a); }
@@ -575,6 +652,7 @@ struct FpOps {
 struct Fp2Ops {
     using F = Fp2;
     static constexpr int WORDS = 2 * NL;
+    static constexpr bool RAW = false;
     __device__ static __forceinline__ void add(F& r, const F& a, const F& b) { fp2_add(r, a, b); }
     __device__ static __forceinline__ void sub(F& r, const F& a, const F& b) { fp2_sub(r, a, b); }
     __device__ static __forceinline__ void dbl(F& r, const F& a) { fp2_dbl(r, a); }
@@ -682,6 +760,7 @@ __device__ __forceinline__ void fp2_lane_mul_sub(Fp& r, const Fp& a, const Fp& a
 struct Fp2LaneOps {
     using F = Fp;
     static constexpr int WORDS = NL;
+    static constexpr bool RAW = false;
     __device__ static __forceinline__ void add(F& r, const F& a, const F& b) { fp_add(r, a, b); }
     __device__ static __forceinline__ void sub(F& r, const F& a, const F& b) { fp_sub(r, a, b); }
     __device__ static __forceinline__ void dbl(F& r, const F& a) { fp_dbl(r, a); }
