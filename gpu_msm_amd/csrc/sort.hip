@@ -50,7 +50,7 @@ hipError_t launch_sort(hipStream_t s, const Plan& p, const uint64_t* d_scalars, 
             k_digits<<<blocks_for(p.n_total, 256), 256, 0, s>>>(desc, p.count, p.n_total, d_scalars, p.L, dig, err);
             GM_TRY(hipGetLastError());
             if (p.stile_total > 0) {
-                k_tile_count<<<p.stile_total, 256, lds, s>>>(desc, p.count, dig, cnt, tbase);
+                k_tile_count<<<p.stile_total, 1024, lds, s>>>(desc, p.count, dig, cnt, tbase);
                 GM_TRY(hipGetLastError());
             }
         } else {
@@ -66,7 +66,7 @@ hipError_t launch_sort(hipStream_t s, const Plan& p, const uint64_t* d_scalars, 
     if (p.n_total > 0) {
         if (tiled) {
             if (p.stile_total > 0) {
-                k_tile_place<<<p.stile_total, 256, lds, s>>>(desc, p.count, dig, off, tbase, sorted);
+                k_tile_place<<<p.stile_total, 1024, lds, s>>>(desc, p.count, dig, off, tbase, sorted);
                 GM_TRY(hipGetLastError());
             }
         } else {

@@ -159,7 +159,7 @@ __device__ __forceinline__ SortTile sort_tile(const MsmDesc& m, uint32_t gt) {
     return t;
 }
 
-__global__ void __launch_bounds__(256) k_tile_count(const MsmDesc* __restrict__ desc, int count,
+__global__ void __launch_bounds__(1024) k_tile_count(const MsmDesc* __restrict__ desc, int count,
                                                     const uint16_t* __restrict__ dig, uint32_t* __restrict__ cnt,
                                                     uint32_t* __restrict__ tbase) {
     extern __shared__ uint32_t lc[];
@@ -182,7 +182,7 @@ __global__ void __launch_bounds__(256) k_tile_count(const MsmDesc* __restrict__ 
     }
 }
 
-__global__ void __launch_bounds__(256) k_tile_place(const MsmDesc* __restrict__ desc, int count,
+__global__ void __launch_bounds__(1024) k_tile_place(const MsmDesc* __restrict__ desc, int count,
                                                     const uint16_t* __restrict__ dig,
                                                     const uint32_t* __restrict__ off,
                                                     const uint32_t* __restrict__ tbase,
