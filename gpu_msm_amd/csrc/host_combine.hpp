@@ -27,24 +27,27 @@ constexpr int words_of() {
     return sizeof(HF) == sizeof(gmh::Fp) ? 16 : 32;
 }
 
-// Horner over nw XYZZ window sums (raw device words), most significant first;
-// false if the sum is the point at infinity.
+// Horner over nw XYZZ window sums (raw device words), most significant first,
+// in Jacobian coordinates (the doublings dominate); false if the sum is the
+// point at infinity.
 template <class HF>
 bool host_combine(HF& x, HF& y, const uint32_t* raw, int nw, int c) {
     constexpr int W = words_of<HF>();
-    auto load = [&](gmh::Xyzz<HF>& p, int w) {
+    auto load = [&](gmh::Jac<HF>& j, int w) {
         const uint32_t* b = raw + (size_t)w * 4 * W;
+        gmh::Xyzz<HF> p;
         dev_elem_to_host(p.X, b);
         dev_elem_to_host(p.Y, b + W);
         dev_elem_to_host(p.ZZ, b + 2 * W);
         dev_elem_to_host(p.ZZZ, b + 3 * W);
+        gmh::from_xyzz(j, p);
     };
-    gmh::Xyzz<HF> r, v;
+    gmh::Jac<HF> r, v;
     load(r, nw - 1);
     for (int w = nw - 2; w >= 0; --w) {
-        for (int k = 0; k < c; ++k) gmh::xdbl(r, r);
+        for (int k = 0; k < c; ++k) gmh::jdbl(r, r);
         load(v, w);
-        gmh::xadd(r, r, v);
+        gmh::jadd(r, r, v);
     }
     return gmh::to_affine(x, y, r);
 }
