@@ -83,28 +83,387 @@ inline void neg(Fp& r, const Fp& a) {
     Fp z = {{0}};
     sub(r, z, a);
 }
-// CIOS Montgomery product, canonical output. p's top word is below 2^62, so
-// the running value never needs the two extra words of textbook CIOS: with
-// inputs below p each row's result stays below 2p in seven words (the
-// "no-carry" variant), and one conditional subtraction ends it.
+// CIOS Montgomery product, canonical output, in x86-64 assembly (BMI2 mulx,
+// ADX adcx/adox: two independent carry chains per row). p's top word is below
+// 2^62, so the running value never needs the two extra words of textbook CIOS
+// (the "no-carry" variant): with inputs below p each row's sum fits eight
+// words (t0..t6 and A), every row's result stays below 2p in seven words, and
+// one conditional subtraction ends it. Registers r8..r15 hold t and A; after
+// each row t0 is zero, so the names rotate by one instead of shifting words:
+// in row i, t_j is r[8 + (i + j) mod 8] and A is r[8 + (i + 7) mod 8].
 inline void mul(Fp& r, const Fp& a, const Fp& b) {
-    uint64_t t[7] = {0};
-    for (int i = 0; i < 7; ++i) {
-        u128 A = (u128)a.l[0] * b.l[i] + t[0];
-        t[0] = (uint64_t)A;
-        uint64_t ah = (uint64_t)(A >> 64);
-        const uint64_t m = t[0] * PINV;
-        u128 C = (u128)m * P.l[0] + t[0];
-        uint64_t ch = (uint64_t)(C >> 64);
-        for (int j = 1; j < 7; ++j) {
-            A = (u128)a.l[j] * b.l[i] + t[j] + ah;
-            ah = (uint64_t)(A >> 64);
-            C = (u128)m * P.l[j] + (uint64_t)A + ch;
-            t[j - 1] = (uint64_t)C;
-            ch = (uint64_t)(C >> 64);
-        }
-        t[6] = ch + ah;
-    }
+    register uint64_t x8 asm("r8"), x9 asm("r9"), x10 asm("r10"), x11 asm("r11"), x12 asm("r12"),
+        x13 asm("r13"), x14 asm("r14"), x15 asm("r15");
+    asm volatile(
+        "xorq %%r8, %%r8\n\t"
+        "xorq %%r9, %%r9\n\t"
+        "xorq %%r10, %%r10\n\t"
+        "xorq %%r11, %%r11\n\t"
+        "xorq %%r12, %%r12\n\t"
+        "xorq %%r13, %%r13\n\t"
+        "xorq %%r14, %%r14\n\t"
+        "xorq %%r15, %%r15\n\t"
+        "movq 0(%[b]), %%rdx\n\t"
+        "xorl %%eax, %%eax\n\t"
+        "mulxq 0(%[a]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r8\n\t"
+        "adcxq %%rbx, %%r9\n\t"
+        "mulxq 8(%[a]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r9\n\t"
+        "adcxq %%rbx, %%r10\n\t"
+        "mulxq 16(%[a]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r10\n\t"
+        "adcxq %%rbx, %%r11\n\t"
+        "mulxq 24(%[a]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r11\n\t"
+        "adcxq %%rbx, %%r12\n\t"
+        "mulxq 32(%[a]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r12\n\t"
+        "adcxq %%rbx, %%r13\n\t"
+        "mulxq 40(%[a]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r13\n\t"
+        "adcxq %%rbx, %%r14\n\t"
+        "mulxq 48(%[a]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r14\n\t"
+        "adcxq %%rbx, %%r15\n\t"
+        "movl $0, %%eax\n\t"
+        "adoxq %%rax, %%r15\n\t"
+        "movq %%r8, %%rdx\n\t"
+        "imulq %[pinv], %%rdx\n\t"
+        "xorl %%eax, %%eax\n\t"
+        "mulxq 0(%[p]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r8\n\t"
+        "adcxq %%rbx, %%r9\n\t"
+        "mulxq 8(%[p]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r9\n\t"
+        "adcxq %%rbx, %%r10\n\t"
+        "mulxq 16(%[p]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r10\n\t"
+        "adcxq %%rbx, %%r11\n\t"
+        "mulxq 24(%[p]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r11\n\t"
+        "adcxq %%rbx, %%r12\n\t"
+        "mulxq 32(%[p]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r12\n\t"
+        "adcxq %%rbx, %%r13\n\t"
+        "mulxq 40(%[p]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r13\n\t"
+        "adcxq %%rbx, %%r14\n\t"
+        "mulxq 48(%[p]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r14\n\t"
+        "adcxq %%rbx, %%r15\n\t"
+        "movl $0, %%eax\n\t"
+        "adoxq %%rax, %%r15\n\t"
+        "movq 8(%[b]), %%rdx\n\t"
+        "xorl %%eax, %%eax\n\t"
+        "mulxq 0(%[a]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r9\n\t"
+        "adcxq %%rbx, %%r10\n\t"
+        "mulxq 8(%[a]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r10\n\t"
+        "adcxq %%rbx, %%r11\n\t"
+        "mulxq 16(%[a]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r11\n\t"
+        "adcxq %%rbx, %%r12\n\t"
+        "mulxq 24(%[a]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r12\n\t"
+        "adcxq %%rbx, %%r13\n\t"
+        "mulxq 32(%[a]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r13\n\t"
+        "adcxq %%rbx, %%r14\n\t"
+        "mulxq 40(%[a]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r14\n\t"
+        "adcxq %%rbx, %%r15\n\t"
+        "mulxq 48(%[a]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r15\n\t"
+        "adcxq %%rbx, %%r8\n\t"
+        "movl $0, %%eax\n\t"
+        "adoxq %%rax, %%r8\n\t"
+        "movq %%r9, %%rdx\n\t"
+        "imulq %[pinv], %%rdx\n\t"
+        "xorl %%eax, %%eax\n\t"
+        "mulxq 0(%[p]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r9\n\t"
+        "adcxq %%rbx, %%r10\n\t"
+        "mulxq 8(%[p]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r10\n\t"
+        "adcxq %%rbx, %%r11\n\t"
+        "mulxq 16(%[p]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r11\n\t"
+        "adcxq %%rbx, %%r12\n\t"
+        "mulxq 24(%[p]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r12\n\t"
+        "adcxq %%rbx, %%r13\n\t"
+        "mulxq 32(%[p]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r13\n\t"
+        "adcxq %%rbx, %%r14\n\t"
+        "mulxq 40(%[p]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r14\n\t"
+        "adcxq %%rbx, %%r15\n\t"
+        "mulxq 48(%[p]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r15\n\t"
+        "adcxq %%rbx, %%r8\n\t"
+        "movl $0, %%eax\n\t"
+        "adoxq %%rax, %%r8\n\t"
+        "movq 16(%[b]), %%rdx\n\t"
+        "xorl %%eax, %%eax\n\t"
+        "mulxq 0(%[a]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r10\n\t"
+        "adcxq %%rbx, %%r11\n\t"
+        "mulxq 8(%[a]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r11\n\t"
+        "adcxq %%rbx, %%r12\n\t"
+        "mulxq 16(%[a]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r12\n\t"
+        "adcxq %%rbx, %%r13\n\t"
+        "mulxq 24(%[a]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r13\n\t"
+        "adcxq %%rbx, %%r14\n\t"
+        "mulxq 32(%[a]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r14\n\t"
+        "adcxq %%rbx, %%r15\n\t"
+        "mulxq 40(%[a]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r15\n\t"
+        "adcxq %%rbx, %%r8\n\t"
+        "mulxq 48(%[a]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r8\n\t"
+        "adcxq %%rbx, %%r9\n\t"
+        "movl $0, %%eax\n\t"
+        "adoxq %%rax, %%r9\n\t"
+        "movq %%r10, %%rdx\n\t"
+        "imulq %[pinv], %%rdx\n\t"
+        "xorl %%eax, %%eax\n\t"
+        "mulxq 0(%[p]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r10\n\t"
+        "adcxq %%rbx, %%r11\n\t"
+        "mulxq 8(%[p]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r11\n\t"
+        "adcxq %%rbx, %%r12\n\t"
+        "mulxq 16(%[p]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r12\n\t"
+        "adcxq %%rbx, %%r13\n\t"
+        "mulxq 24(%[p]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r13\n\t"
+        "adcxq %%rbx, %%r14\n\t"
+        "mulxq 32(%[p]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r14\n\t"
+        "adcxq %%rbx, %%r15\n\t"
+        "mulxq 40(%[p]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r15\n\t"
+        "adcxq %%rbx, %%r8\n\t"
+        "mulxq 48(%[p]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r8\n\t"
+        "adcxq %%rbx, %%r9\n\t"
+        "movl $0, %%eax\n\t"
+        "adoxq %%rax, %%r9\n\t"
+        "movq 24(%[b]), %%rdx\n\t"
+        "xorl %%eax, %%eax\n\t"
+        "mulxq 0(%[a]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r11\n\t"
+        "adcxq %%rbx, %%r12\n\t"
+        "mulxq 8(%[a]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r12\n\t"
+        "adcxq %%rbx, %%r13\n\t"
+        "mulxq 16(%[a]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r13\n\t"
+        "adcxq %%rbx, %%r14\n\t"
+        "mulxq 24(%[a]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r14\n\t"
+        "adcxq %%rbx, %%r15\n\t"
+        "mulxq 32(%[a]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r15\n\t"
+        "adcxq %%rbx, %%r8\n\t"
+        "mulxq 40(%[a]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r8\n\t"
+        "adcxq %%rbx, %%r9\n\t"
+        "mulxq 48(%[a]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r9\n\t"
+        "adcxq %%rbx, %%r10\n\t"
+        "movl $0, %%eax\n\t"
+        "adoxq %%rax, %%r10\n\t"
+        "movq %%r11, %%rdx\n\t"
+        "imulq %[pinv], %%rdx\n\t"
+        "xorl %%eax, %%eax\n\t"
+        "mulxq 0(%[p]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r11\n\t"
+        "adcxq %%rbx, %%r12\n\t"
+        "mulxq 8(%[p]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r12\n\t"
+        "adcxq %%rbx, %%r13\n\t"
+        "mulxq 16(%[p]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r13\n\t"
+        "adcxq %%rbx, %%r14\n\t"
+        "mulxq 24(%[p]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r14\n\t"
+        "adcxq %%rbx, %%r15\n\t"
+        "mulxq 32(%[p]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r15\n\t"
+        "adcxq %%rbx, %%r8\n\t"
+        "mulxq 40(%[p]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r8\n\t"
+        "adcxq %%rbx, %%r9\n\t"
+        "mulxq 48(%[p]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r9\n\t"
+        "adcxq %%rbx, %%r10\n\t"
+        "movl $0, %%eax\n\t"
+        "adoxq %%rax, %%r10\n\t"
+        "movq 32(%[b]), %%rdx\n\t"
+        "xorl %%eax, %%eax\n\t"
+        "mulxq 0(%[a]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r12\n\t"
+        "adcxq %%rbx, %%r13\n\t"
+        "mulxq 8(%[a]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r13\n\t"
+        "adcxq %%rbx, %%r14\n\t"
+        "mulxq 16(%[a]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r14\n\t"
+        "adcxq %%rbx, %%r15\n\t"
+        "mulxq 24(%[a]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r15\n\t"
+        "adcxq %%rbx, %%r8\n\t"
+        "mulxq 32(%[a]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r8\n\t"
+        "adcxq %%rbx, %%r9\n\t"
+        "mulxq 40(%[a]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r9\n\t"
+        "adcxq %%rbx, %%r10\n\t"
+        "mulxq 48(%[a]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r10\n\t"
+        "adcxq %%rbx, %%r11\n\t"
+        "movl $0, %%eax\n\t"
+        "adoxq %%rax, %%r11\n\t"
+        "movq %%r12, %%rdx\n\t"
+        "imulq %[pinv], %%rdx\n\t"
+        "xorl %%eax, %%eax\n\t"
+        "mulxq 0(%[p]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r12\n\t"
+        "adcxq %%rbx, %%r13\n\t"
+        "mulxq 8(%[p]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r13\n\t"
+        "adcxq %%rbx, %%r14\n\t"
+        "mulxq 16(%[p]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r14\n\t"
+        "adcxq %%rbx, %%r15\n\t"
+        "mulxq 24(%[p]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r15\n\t"
+        "adcxq %%rbx, %%r8\n\t"
+        "mulxq 32(%[p]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r8\n\t"
+        "adcxq %%rbx, %%r9\n\t"
+        "mulxq 40(%[p]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r9\n\t"
+        "adcxq %%rbx, %%r10\n\t"
+        "mulxq 48(%[p]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r10\n\t"
+        "adcxq %%rbx, %%r11\n\t"
+        "movl $0, %%eax\n\t"
+        "adoxq %%rax, %%r11\n\t"
+        "movq 40(%[b]), %%rdx\n\t"
+        "xorl %%eax, %%eax\n\t"
+        "mulxq 0(%[a]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r13\n\t"
+        "adcxq %%rbx, %%r14\n\t"
+        "mulxq 8(%[a]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r14\n\t"
+        "adcxq %%rbx, %%r15\n\t"
+        "mulxq 16(%[a]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r15\n\t"
+        "adcxq %%rbx, %%r8\n\t"
+        "mulxq 24(%[a]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r8\n\t"
+        "adcxq %%rbx, %%r9\n\t"
+        "mulxq 32(%[a]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r9\n\t"
+        "adcxq %%rbx, %%r10\n\t"
+        "mulxq 40(%[a]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r10\n\t"
+        "adcxq %%rbx, %%r11\n\t"
+        "mulxq 48(%[a]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r11\n\t"
+        "adcxq %%rbx, %%r12\n\t"
+        "movl $0, %%eax\n\t"
+        "adoxq %%rax, %%r12\n\t"
+        "movq %%r13, %%rdx\n\t"
+        "imulq %[pinv], %%rdx\n\t"
+        "xorl %%eax, %%eax\n\t"
+        "mulxq 0(%[p]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r13\n\t"
+        "adcxq %%rbx, %%r14\n\t"
+        "mulxq 8(%[p]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r14\n\t"
+        "adcxq %%rbx, %%r15\n\t"
+        "mulxq 16(%[p]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r15\n\t"
+        "adcxq %%rbx, %%r8\n\t"
+        "mulxq 24(%[p]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r8\n\t"
+        "adcxq %%rbx, %%r9\n\t"
+        "mulxq 32(%[p]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r9\n\t"
+        "adcxq %%rbx, %%r10\n\t"
+        "mulxq 40(%[p]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r10\n\t"
+        "adcxq %%rbx, %%r11\n\t"
+        "mulxq 48(%[p]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r11\n\t"
+        "adcxq %%rbx, %%r12\n\t"
+        "movl $0, %%eax\n\t"
+        "adoxq %%rax, %%r12\n\t"
+        "movq 48(%[b]), %%rdx\n\t"
+        "xorl %%eax, %%eax\n\t"
+        "mulxq 0(%[a]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r14\n\t"
+        "adcxq %%rbx, %%r15\n\t"
+        "mulxq 8(%[a]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r15\n\t"
+        "adcxq %%rbx, %%r8\n\t"
+        "mulxq 16(%[a]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r8\n\t"
+        "adcxq %%rbx, %%r9\n\t"
+        "mulxq 24(%[a]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r9\n\t"
+        "adcxq %%rbx, %%r10\n\t"
+        "mulxq 32(%[a]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r10\n\t"
+        "adcxq %%rbx, %%r11\n\t"
+        "mulxq 40(%[a]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r11\n\t"
+        "adcxq %%rbx, %%r12\n\t"
+        "mulxq 48(%[a]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r12\n\t"
+        "adcxq %%rbx, %%r13\n\t"
+        "movl $0, %%eax\n\t"
+        "adoxq %%rax, %%r13\n\t"
+        "movq %%r14, %%rdx\n\t"
+        "imulq %[pinv], %%rdx\n\t"
+        "xorl %%eax, %%eax\n\t"
+        "mulxq 0(%[p]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r14\n\t"
+        "adcxq %%rbx, %%r15\n\t"
+        "mulxq 8(%[p]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r15\n\t"
+        "adcxq %%rbx, %%r8\n\t"
+        "mulxq 16(%[p]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r8\n\t"
+        "adcxq %%rbx, %%r9\n\t"
+        "mulxq 24(%[p]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r9\n\t"
+        "adcxq %%rbx, %%r10\n\t"
+        "mulxq 32(%[p]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r10\n\t"
+        "adcxq %%rbx, %%r11\n\t"
+        "mulxq 40(%[p]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r11\n\t"
+        "adcxq %%rbx, %%r12\n\t"
+        "mulxq 48(%[p]), %%rax, %%rbx\n\t"
+        "adoxq %%rax, %%r12\n\t"
+        "adcxq %%rbx, %%r13\n\t"
+        "movl $0, %%eax\n\t"
+        "adoxq %%rax, %%r13\n\t"
+        : "=&r"(x8), "=&r"(x9), "=&r"(x10), "=&r"(x11), "=&r"(x12), "=&r"(x13), "=&r"(x14), "=&r"(x15)
+        : [a] "r"(a.l), [b] "r"(b.l), [p] "r"(P.l), [pinv] "m"(PINV)
+        : "rax", "rbx", "rdx", "cc", "memory");
+    uint64_t t[7] = {x15, x8, x9, x10, x11, x12, x13};
     if (geq_p(t)) sub_p(t);
     memcpy(r.l, t, sizeof r.l);
 }
