@@ -49,13 +49,17 @@ def test_missing_library_fails_loudly(monkeypatch):
 
 
 def test_montgomery_helpers_match_oracle(L):
+    # fp_to_montgomery_wrapper is one host Montgomery product (v * R^2 R^-1):
+    # random values and the edges 0, 1, p - 1, p - 2 and values with every
+    # 64-bit word at its maximum below p pin the host product against Python
     rng = np.random.default_rng(11)
-    for _ in range(20):
-        v = int.from_bytes(rng.bytes(56), "little") % m.P
+    vals = [0, 1, 2, m.P - 1, m.P - 2, (1 << 445), m.P - (1 << 64), (1 << 64) - 1]
+    vals += [int.from_bytes(rng.bytes(56), "little") % m.P for _ in range(2000)]
+    for v in vals:
         a = o.int_to_limbs(v)
         out = np.zeros(7, dtype=np.uint64)
         L.fp_to_montgomery_wrapper(out.ctypes.data, a.ctypes.data)
-        assert o.limbs_to_int(out) == m.to_mont(v)
+        assert o.limbs_to_int(out) == m.to_mont(v), v
     g = o.g1_generator()  # Montgomery
     r = np.zeros(1, dtype=gm.G1_DTYPE)
     L.g1_from_montgomery_wrapper(r.ctypes.data, g.ctypes.data)
