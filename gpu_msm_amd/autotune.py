@@ -26,13 +26,17 @@ CAP_POINTS = 1 << 22
 
 
 def candidates(curve: int, log2n: int):
+    """Plans timed per cell: widths in the band [h - 3, h + 2] around the
+    heuristic's h whose top window is not sparse, times segments 4..32, plus
+    the heuristic's own plan, (0, 0): the library's choice, recorded with
+    seg 0."""
     import gpu_msm_amd as gm
 
     h, _, _ = gm.plan_query(curve, 1 << log2n, 5, 299)
-    cs = [c for c in range(h - 3, h + 3) if pl.C_MIN <= c <= pl.C_MAX]
-    out = []
+    cs = [c for c in range(h - 3, h + 3) if pl.C_MIN <= c <= pl.C_MAX and pl.window_ok(c)]
+    out = [(0, 0)]
     for c in cs:
-        for s in (4, 8, 16):
+        for s in pl.SEGS:
             if s <= (1 << (c - 1)):
                 out.append((c, s))
     return out, h
@@ -52,7 +56,7 @@ def time_cell(gm, bases, scalars, pool_n: int, curve: int, log2n: int, batch: in
     return best * 1e3 / batch
 
 
-def sweep(curves=(1, 2), log2ns=range(8, 21), batches=(1, 8, 64, 256), reps: int = 2, log=print,
+def sweep(curves=(1, 2), log2ns=range(8, 21), batches=(1, 8, 64, 256), reps: int = 3, log=print,
           on_cell=None):
     """Timings of every candidate plan per cell; `on_cell(records)` is called
     after each cell (the caller can checkpoint partial sweeps)."""
@@ -72,9 +76,13 @@ def sweep(curves=(1, 2), log2ns=range(8, 21), batches=(1, 8, 64, 256), reps: int
                 cands, h = candidates(curve, lg)
                 for plan in cands:
                     ms = time_cell(gm, bases, sc, pool_n, curve, lg, b, plan, reps, rng)
-                    records.append({"curve": curve, "log2n": lg, "batch": b, "c": plan[0], "seg": plan[1],
-                                    "ms_per_msm": round(ms, 5), "heuristic_c": h})
-                cell = [r for r in records if r["curve"] == curve and r["log2n"] == lg and r["batch"] == b]
+                    rec = {"curve": curve, "log2n": lg, "batch": b, "c": plan[0] or h, "seg": plan[1],
+                           "ms_per_msm": round(ms, 5), "heuristic_c": h}
+                    if plan == (0, 0):
+                        rec["heuristic"] = True  # the library's own plan (seg chosen per batch)
+                    records.append(rec)
+                cell = [r for r in records if r["curve"] == curve and r["log2n"] == lg and r["batch"] == b
+                        and not r.get("heuristic")]
                 best = min(cell, key=lambda r: r["ms_per_msm"])
                 log(f"autotune: curve {curve} N=2^{lg} batch {b}: best c={best['c']} seg={best['seg']} "
                     f"{best['ms_per_msm']:.4f} ms/MSM over {len(cell)} plans")
@@ -88,7 +96,8 @@ def sweep(curves=(1, 2), log2ns=range(8, 21), batches=(1, 8, 64, 256), reps: int
 def best_table(records):
     cells = {}
     for r in records:
-        cells.setdefault((r["curve"], r["log2n"], r["batch"]), []).append(r)
+        if not r.get("heuristic"):
+            cells.setdefault((r["curve"], r["log2n"], r["batch"]), []).append(r)
     best = []
     for (cv, lg, b), rs in sorted(cells.items()):
         r = min(rs, key=lambda q: q["ms_per_msm"])
@@ -102,7 +111,7 @@ def training_set(records, tol: float = 0.02):
     window are left out: the planner never picks them."""
     cells = {}
     for r in records:
-        if pl.window_ok(r["c"]):
+        if pl.window_ok(r["c"]) and not r.get("heuristic"):
             cells.setdefault((r["curve"], r["log2n"], r["batch"]), []).append(r)
     xs, ts = [], []
     for (cv, lg, b), rs in sorted(cells.items()):

@@ -49,7 +49,7 @@ struct MsmDesc {
     uint32_t nw;         // windows
     uint32_t B;          // buckets per window = 2^(c-1)
     uint32_t seg;        // buckets per bucket-reduce lane
-    uint32_t nseg;       // segments per window = B / seg
+    uint32_t nseg;       // segments per window = ceil(B / seg); the last may be shorter
     uint32_t bkt_base;   // first global bucket
     uint32_t pt_base;    // first flattened point
     uint32_t rank_base;  // first (window, point) rank slot

@@ -398,16 +398,17 @@ __global__ void __launch_bounds__(64) k_bucket_reduce(const MsmDesc* __restrict_
     const uint32_t local = g - m.seg_base;
     const uint32_t w = local / m.nseg, t = local - w * m.nseg;
     const typename Stor<O>::X* bw = buckets + m.bkt_base + (size_t)w * m.B + (size_t)t * m.seg;
+    const uint32_t lo = t * m.seg;
+    const int len = (int)min(m.seg, m.B - lo);  // the window's last segment may be shorter
     Xyzz<O> S, T;
     xyzz_set_inf(S);
     xyzz_set_inf(T);
-    for (int b = (int)m.seg - 1; b >= 0; --b) {
+    for (int b = len - 1; b >= 0; --b) {
         Xyzz<O> bk;
         ldp(bk, &bw[b]);
         xyzz_add<O>(S, S, bk);
         xyzz_add<O>(T, T, S);
     }
-    const uint32_t lo = t * m.seg;
     if (lo != 0 && !xyzz_is_inf(S)) {
         Xyzz<O> M;
         xyzz_set_inf(M);

@@ -71,13 +71,13 @@ bool make_plan(Plan& p, MsmDesc* d, int curve, int L, int bits, const MsmSpec* s
     p.count = count;
     p.host_desc = d;
     // Buckets per bucket-reduction lane for the MSMs that leave it to the
-    // plan: the smallest power of two >= 4 that keeps the batch's reduction
-    // within REDUCE_THREADS threads (G2 runs two threads per lane), at most
-    // 16. Fewer, longer lanes leave SIMDs idle; more, shorter ones add rounds
-    // and the per-segment scalar multiples. The sweeps put the optimum at this
-    // size for both curves (seg 8 for G1 and 16 for G2 at N = 2^20, 4 for
-    // small MSMs).
-    constexpr uint64_t REDUCE_THREADS = 48 * 1024;
+    // plan: the smallest count >= 4 that keeps the batch's reduction within
+    // REDUCE_THREADS threads (G2 runs two threads per lane), i.e. about one
+    // wave per SIMD (the reduction runs at one wave per SIMD, so a second
+    // round would double its time, while fewer waves leave SIMDs idle and
+    // lengthen each lane's running sums). Small MSMs get 4, the sweeps'
+    // optimum for them; at N = 2^20 G1 gets 6 and G2 11.
+    constexpr uint64_t REDUCE_THREADS = 62 * 1024;
     const uint64_t lpi = curve == CURVE_G2 ? 2 : 1;
     uint64_t auto_bkt = 0, fixed_thr = 0;
     for (int j = 0; j < count; ++j) {
@@ -85,13 +85,18 @@ bool make_plan(Plan& p, MsmDesc* d, int curve, int L, int bits, const MsmSpec* s
         const int c = s.c > 0 ? s.c : choose_window(curve, s.n, bits);
         if (c < 2 || c > 24) return false;
         const uint64_t B = 1ull << (c - 1), nbk = B * (uint64_t)windows_for(bits, c);
-        if (s.seg > 0)
-            fixed_thr += nbk / std::min<uint64_t>(B, (uint64_t)s.seg) * lpi;
-        else
+        if (s.seg > 0) {
+            const uint64_t sg = std::min<uint64_t>(B, (uint64_t)s.seg);
+            fixed_thr += (B + sg - 1) / sg * windows_for(bits, c) * lpi;
+        } else {
             auto_bkt += nbk;
+        }
     }
-    uint32_t auto_seg = 4;
-    while (auto_seg < 16 && fixed_thr + auto_bkt * lpi / auto_seg > REDUCE_THREADS) auto_seg <<= 1;
+    uint64_t auto_seg = 4;
+    if (fixed_thr < REDUCE_THREADS)
+        auto_seg = std::max<uint64_t>(auto_seg, (auto_bkt * lpi + (REDUCE_THREADS - fixed_thr) - 1) /
+                                                    (REDUCE_THREADS - fixed_thr));
+    auto_seg = std::min<uint64_t>(auto_seg, 32);
     uint64_t pt = 0, rank = 0, bkt = 0, seg = 0, win = 0, tile = 0;
     for (int j = 0; j < count; ++j) {
         const MsmSpec& s = specs[j];
@@ -102,7 +107,6 @@ bool make_plan(Plan& p, MsmDesc* d, int curve, int L, int bits, const MsmSpec* s
         const int B = 1 << (c - 1);
         int sg = s.seg > 0 ? s.seg : (int)auto_seg;
         if (sg > B) sg = B;
-        if ((sg & (sg - 1)) != 0) return false;  // power of two dividing B
         MsmDesc& m = d[j];
         memset(&m, 0, sizeof m);
         m.n = (uint32_t)s.n;
@@ -112,7 +116,7 @@ bool make_plan(Plan& p, MsmDesc* d, int curve, int L, int bits, const MsmSpec* s
         m.nw = (uint32_t)nw;
         m.B = (uint32_t)B;
         m.seg = (uint32_t)sg;
-        m.nseg = (uint32_t)(B / sg);
+        m.nseg = (uint32_t)((B + sg - 1) / sg);  // the last segment may be shorter
         m.bkt_base = (uint32_t)bkt;
         m.pt_base = (uint32_t)pt;
         m.rank_base = (uint32_t)rank;
@@ -122,7 +126,7 @@ bool make_plan(Plan& p, MsmDesc* d, int curve, int L, int bits, const MsmSpec* s
         pt += (uint64_t)s.n;
         rank += (uint64_t)nw * s.n;
         bkt += (uint64_t)nw * B;
-        seg += (uint64_t)nw * (B / sg);
+        seg += (uint64_t)nw * m.nseg;
         win += (uint64_t)nw;
         tile += (uint64_t)nw * ((m.nseg + WSUM_TILE - 1) / WSUM_TILE);
         if (rank >= 0xFFFFFFFFull || bkt >= 0x7FFFFFFFull) return false;

@@ -29,7 +29,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 TUNING_DIR = os.path.join(HERE, "tuning")
 
 C_MIN, C_MAX = 3, 20
-SEGS = (2, 4, 8, 16, 32)
+SEGS = (4, 8, 16, 32)
 
 
 def features(log2n, batch, curve) -> np.ndarray:
@@ -61,7 +61,7 @@ def decode(y: np.ndarray):
             out.append(None)
             continue
         cc = min(max(cc, C_MIN), C_MAX)
-        seg = 2 ** int(round(min(max(li, 1.0), 5.0)))
+        seg = 2 ** int(round(min(max(li, 2.0), 5.0)))
         seg = min(seg, 1 << (cc - 1))
         out.append((cc, seg))
     return out

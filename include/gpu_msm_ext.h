@@ -72,7 +72,7 @@ typedef struct gm_msm_spec {
     int base_offset;
     int scalar_offset;
     int window; /* window bits c */
-    int seg;    /* buckets per bucket-reduction lane (power of two) */
+    int seg;    /* buckets per bucket-reduction lane (any count >= 1; capped at 2^(window-1)) */
 } gm_msm_spec;
 
 /* Many MSMs of uneven sizes in one pipeline run: every stage is one launch

@@ -169,6 +169,19 @@ def test_g1_2_16_full_and_windows(golden, g1_pool_16):
     assert same(sub, o.msm("g1", o.bases_from_k("g1", ks[4096:5120]), ss[4096:5120]))
 
 
+@pytest.mark.parametrize("seg", [1, 3, 5, 6, 7, 11, 33, 100000])
+def test_bucket_segments_any_length(g1_pool_16, seg):
+    """Bucket-reduction segments of any length, including ones that do not
+    divide 2^(c-1) (the window's last segment is shorter) and one longer than
+    the window (capped at 2^(c-1)), give the same result as the default plan,
+    for two windows."""
+    ks, ss, b, s = g1_pool_16
+    for c, n in ((9, 3000), (13, 65536)):
+        ref = gm.msm_batch(b, s, [(n, 0, 0, c, 0)], scalar_bits=299)
+        got = gm.msm_batch(b, s, [(n, 0, 0, c, seg)], scalar_bits=299)
+        assert same(got, ref), (seg, c)
+
+
 def test_g2_2_14_closed_form_with_zeros(golden):
     w = golden["g2_2_14"]
     n = w["n"]
