@@ -59,6 +59,11 @@ hipError_t launch_window_sum(hipStream_t s, const Plan& p, uint8_t* ws) {
     using X = typename Stor<K>::X;
     if (p.win_total == 0) return hipSuccess;
     const MsmDesc* desc = reinterpret_cast<const MsmDesc*>(ws + p.o_desc);
+    // windows with few segments: one lane each (MSMs flagged DESC_WSUM_DIRECT)
+    k_window_direct<K><<<blocks_for((long)p.win_total * Stor<K>::LPI, 64), 64, 0, s>>>(
+        desc, p.count, p.win_total, reinterpret_cast<const X*>(ws + p.o_segs), reinterpret_cast<X*>(ws + p.o_wsum));
+    GM_TRY(hipGetLastError());
+    if (p.tile_total == 0) return hipSuccess;  // every window was summed directly
     k_window_partial<K><<<p.tile_total, 64 * Stor<K>::LPI, 0, s>>>(
         desc, p.count, reinterpret_cast<const X*>(ws + p.o_segs), reinterpret_cast<X*>(ws + p.o_tiles));
     GM_TRY(hipGetLastError());

@@ -61,13 +61,19 @@ struct MsmDesc {
     uint32_t stile_base; // first sort tile (stile_pts points of one window)
     uint32_t stile_pts;  // points per sort tile
     uint32_t tb_base;    // first per-tile bucket base (B words per sort tile)
-    uint32_t pad;
+    uint32_t flags;      // DESC_WSUM_DIRECT: window sums by one lane per window
 };
 static_assert(sizeof(MsmDesc) == 80, "MsmDesc layout");
 
 // Segments summed by one window-sum tile (64 threads x 4) before the
 // per-window total over tiles.
 constexpr uint32_t WSUM_TILE = 256;
+// Windows of at most 4 segments, or of at most WSUM_DIRECT_MAX in batches of
+// many windows, are summed by one lane each (k_window_direct): a sequential
+// sum of a few segments is no deeper than the two tree stages, and a batch
+// of many small MSMs then runs one lane, not one workgroup, per window.
+constexpr uint32_t WSUM_DIRECT_MAX = 24;
+constexpr uint32_t DESC_WSUM_DIRECT = 1u;
 
 // Counting sort with per-tile bucket counters in LDS: used while every MSM of
 // a batch has c <= SORT_LDS_MAX_C (2^(c-1) counters of 4 bytes per

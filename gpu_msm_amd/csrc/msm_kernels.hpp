@@ -451,7 +451,11 @@ __global__ void __launch_bounds__(64 * Stor<O>::LPI) k_window_partial(const MsmD
     }
     sh[tid] = acc;
     __syncthreads();
-    for (int s = T / 2; s >= Stor<O>::LPI; s >>= 1) {
+    // tree over the items that hold segments only (uniform over the block)
+    const uint32_t items = min((uint32_t)(T / Stor<O>::LPI), t1 - k * WSUM_TILE);
+    uint32_t span = 1;
+    while (span < items) span <<= 1;
+    for (int s = (int)(span / 2) * Stor<O>::LPI; s >= Stor<O>::LPI; s >>= 1) {
         if (tid < s) {
             Xyzz<O> a = sh[tid], b = sh[tid + s];
             xyzz_add<O>(a, a, b);
@@ -470,6 +474,7 @@ __global__ void __launch_bounds__(THREADS) k_window_sum(const MsmDesc* __restric
     const uint32_t gw = blockIdx.x;
     const int tid = threadIdx.x;
     const MsmDesc& m = desc[GM_FIND(desc, count, win_base, gw)];
+    if (m.flags & DESC_WSUM_DIRECT) return;  // k_window_direct sums this window
     const uint32_t w = gw - m.win_base;
     const uint32_t ntile = (m.nseg + WSUM_TILE - 1) / WSUM_TILE;
     const typename Stor<O>::X* tw = tiles + m.tile_base + (size_t)w * ntile;
@@ -482,7 +487,10 @@ __global__ void __launch_bounds__(THREADS) k_window_sum(const MsmDesc* __restric
     }
     sh[tid] = acc;
     __syncthreads();
-    for (int s = THREADS / 2; s >= Stor<O>::LPI; s >>= 1) {
+    const uint32_t items = min((uint32_t)(THREADS / Stor<O>::LPI), ntile);
+    uint32_t span = 1;
+    while (span < items) span <<= 1;
+    for (int s = (int)(span / 2) * Stor<O>::LPI; s >= Stor<O>::LPI; s >>= 1) {
         if (tid < s) {
             Xyzz<O> a = sh[tid], b = sh[tid + s];
             xyzz_add<O>(a, a, b);
@@ -491,6 +499,27 @@ __global__ void __launch_bounds__(THREADS) k_window_sum(const MsmDesc* __restric
         __syncthreads();
     }
     if (tid < Stor<O>::LPI) stp(&wsum[gw], sh[tid]);
+}
+
+// Windows with few segments (DESC_WSUM_DIRECT): one lane per window adds its
+// segment results in order.
+template <class O>
+__global__ void __launch_bounds__(64) k_window_direct(const MsmDesc* __restrict__ desc, int count, uint32_t win_total,
+                                                      const typename Stor<O>::X* __restrict__ segs,
+                                                      typename Stor<O>::X* __restrict__ wsum) {
+    const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / Stor<O>::LPI;
+    if (gw >= win_total) return;
+    const MsmDesc& m = desc[GM_FIND(desc, count, win_base, gw)];
+    if (!(m.flags & DESC_WSUM_DIRECT)) return;
+    const typename Stor<O>::X* sw = segs + m.seg_base + (size_t)(gw - m.win_base) * m.nseg;
+    Xyzz<O> acc;
+    ldp(acc, &sw[0]);
+    for (uint32_t t = 1; t < m.nseg; ++t) {
+        Xyzz<O> v;
+        ldp(v, &sw[t]);
+        xyzz_add<O>(acc, acc, v);
+    }
+    stp(&wsum[gw], acc);
 }
 
 // Horner over window sums on the device, one lane per MSM; writes the

@@ -122,14 +122,23 @@ bool make_plan(Plan& p, MsmDesc* d, int curve, int L, int bits, const MsmSpec* s
         m.rank_base = (uint32_t)rank;
         m.seg_base = (uint32_t)seg;
         m.win_base = (uint32_t)win;
-        m.tile_base = (uint32_t)tile;
         pt += (uint64_t)s.n;
         rank += (uint64_t)nw * s.n;
         bkt += (uint64_t)nw * B;
         seg += (uint64_t)nw * m.nseg;
         win += (uint64_t)nw;
-        tile += (uint64_t)nw * ((m.nseg + WSUM_TILE - 1) / WSUM_TILE);
         if (rank >= 0xFFFFFFFFull || bkt >= 0x7FFFFFFFull) return false;
+    }
+    // Window sums: one lane per window (k_window_direct) for windows of at
+    // most 4 segments, and up to WSUM_DIRECT_MAX when the batch has so many
+    // windows that one workgroup per window would need more than one round
+    // of the chip's SIMDs; otherwise tiles and a tree per window.
+    for (int j = 0; j < count; ++j) {
+        MsmDesc& m = d[j];
+        const bool direct = m.nseg <= 4 || (m.nseg <= WSUM_DIRECT_MAX && win >= 1024);
+        m.flags = direct ? DESC_WSUM_DIRECT : 0u;
+        m.tile_base = (uint32_t)tile;
+        if (!direct) tile += (uint64_t)m.nw * ((m.nseg + WSUM_TILE - 1) / WSUM_TILE);
     }
     // Sort tiles: every MSM of the batch must fit its counters in LDS.
     bool lds_sort = true;

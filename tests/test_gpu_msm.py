@@ -270,6 +270,33 @@ def test_batch_matches_alone_and_oracle(curve):
             assert same(got[k:k + 1], want), k
 
 
+@pytest.mark.parametrize("curve", ["g1", "g2"])
+def test_batch_many_windows_direct_window_sums(curve):
+    """A batch of over 1024 windows sums windows of few segments with one
+    lane each, and the rest with tiles; every MSM equals the same MSM run
+    alone (which takes the tile path) and, for a sample, the oracle."""
+    pool_n = 1 << 13
+    ks, b = _pool(curve, pool_n, 110 if curve == "g1" else 111)
+    rng = np.random.default_rng(12)
+    sizes = [300] * 40 + [4096, 50, 2000]
+    windows = [6] * 40 + [12, 5, 10]
+    ss = o.gen_scalars_modr(112, sum(sizes))
+    s = gm.Scalars.upload(ss)
+    specs, so = [], 0
+    for n, w in zip(sizes, windows):
+        specs.append((n, int(rng.integers(0, pool_n - n + 1)), so, w))
+        so += n
+    got = gm.msm_batch(b, s, specs, scalar_bits=299)
+    for k in (0, 17, 39, 40, 41, 42):
+        n, bo, so_, w = specs[k]
+        alone = gm.msm_resident(b, s, n=n, base_offset=bo, scalar_offset=so_, scalar_bits=299, window=w)
+        assert same(got[k:k + 1], alone), k
+    for k in (3, 41):
+        n, bo, so_, _ = specs[k]
+        want = o.msm(curve, o.bases_from_k(curve, ks[bo:bo + n]), ss[so_:so_ + n])
+        assert same(got[k:k + 1], want), k
+
+
 def test_batch_u64_scalars_and_reorder():
     """Permuting the MSMs of a batch does not change any result."""
     pool_n = 4096
