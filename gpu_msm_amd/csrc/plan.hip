@@ -75,8 +75,9 @@ bool make_plan(Plan& p, MsmDesc* d, int curve, int L, int bits, const MsmSpec* s
     // REDUCE_THREADS threads (G2 runs two threads per lane), i.e. about one
     // wave per SIMD (the reduction runs at one wave per SIMD, so a second
     // round would double its time, while fewer waves leave SIMDs idle and
-    // lengthen each lane's running sums). Small MSMs get 4, the sweeps'
-    // optimum for them; at N = 2^20 G1 gets 6 and G2 11.
+    // lengthen each lane's running sums), capped where a batch needs more
+    // than one round. Small MSMs get 4, the sweeps' optimum for them; at
+    // N = 2^20 G1 gets 6 and G2 11.
     constexpr uint64_t REDUCE_THREADS = 62 * 1024;
     const uint64_t lpi = curve == CURVE_G2 ? 2 : 1;
     uint64_t auto_bkt = 0, fixed_thr = 0;
@@ -96,7 +97,8 @@ bool make_plan(Plan& p, MsmDesc* d, int curve, int L, int bits, const MsmSpec* s
     if (fixed_thr < REDUCE_THREADS)
         auto_seg = std::max<uint64_t>(auto_seg, (auto_bkt * lpi + (REDUCE_THREADS - fixed_thr) - 1) /
                                                     (REDUCE_THREADS - fixed_thr));
-    auto_seg = std::min<uint64_t>(auto_seg, 32);
+    // beyond one round of waves the sweeps favour 16 for G1 and 32 for G2
+    auto_seg = std::min<uint64_t>(auto_seg, curve == CURVE_G2 ? 32 : 16);
     uint64_t pt = 0, rank = 0, bkt = 0, seg = 0, win = 0, tile = 0;
     for (int j = 0; j < count; ++j) {
         const MsmSpec& s = specs[j];
