@@ -15,7 +15,8 @@
 //   k_bucket_reduce one lane per segment of buckets: running sums give
 //                  sum_b (b+1) B_b for the segment as T + lo * S
 //   k_window_partial / k_window_sum  tiles of segments, then each window's
-//                  total over its tiles (two short tree stages)
+//                  total over its tiles (two short tree stages); windows of
+//                  few segments: k_window_direct, one lane per window
 //   combine        Horner over windows, most significant first (reference
 //                  src/curve.cu:2060-2179 combines on the CPU after every
 //                  window; here on the host once, or k_horner on the device)
@@ -425,9 +426,11 @@ __global__ void __launch_bounds__(64) k_bucket_reduce(const MsmDesc* __restrict_
 // Window sums in two short stages, so the serial depth stays small and the
 // whole GPU takes part (one workgroup per window would leave most CUs idle):
 //   k_window_partial  one 64-thread workgroup per tile of WSUM_TILE segments
-//                     of one window: 4 strided adds per thread, then a tree
-//                     in LDS;
+//                     of one window: up to 4 strided adds per thread, then a
+//                     tree in LDS over the items that hold segments;
 //   k_window_sum      one workgroup per (MSM, window): tree over its tiles.
+// Windows of MSMs flagged DESC_WSUM_DIRECT (few segments) skip both and are
+// summed by k_window_direct, one lane per window.
 template <class O>
 __global__ void __launch_bounds__(64 * Stor<O>::LPI) k_window_partial(const MsmDesc* __restrict__ desc, int count,
                                                        const typename Stor<O>::X* __restrict__ segs,
