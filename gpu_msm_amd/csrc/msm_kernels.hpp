@@ -371,7 +371,11 @@ __global__ void __launch_bounds__(THREADS) k_bucket_fixup_heavy(const MsmDesc* _
         }
         sh[tid] = acc;
         __syncthreads();
-        for (int s = THREADS / 2; s >= Stor<O>::LPI; s >>= 1) {
+        // tree over the items that hold parts (uniform over the block)
+        const uint32_t items = min((uint32_t)(THREADS / Stor<O>::LPI), parts);
+        uint32_t span = 1;
+        while (span < items) span <<= 1;
+        for (int s = (int)(span / 2) * Stor<O>::LPI; s >= Stor<O>::LPI; s >>= 1) {
             if (tid < s) {
                 Xyzz<O> a = sh[tid], b = sh[tid + s];
                 xyzz_add<O>(a, a, b);
