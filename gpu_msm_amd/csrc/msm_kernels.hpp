@@ -341,6 +341,29 @@ __global__ void __launch_bounds__(64) k_bucket_fixup(const MsmDesc* __restrict__
     stp(&buckets[g], acc);
 }
 
+// Sums sh[0..T) into sh[0..LPI) by a tree in LDS. The first `items` items
+// (LPI threads each) hold values, the rest O, so levels above the first power
+// of two >= items are skipped; the test is uniform over the block, and the
+// levels are unrolled at compile time. The caller has stored its values and
+// synchronised.
+template <class O, int T>
+__device__ __forceinline__ void lds_tree(Xyzz<O>* sh, int tid, uint32_t items) {
+    uint32_t span = 1;
+    while (span < items) span <<= 1;
+    const int lim = (int)span * Stor<O>::LPI;
+#pragma unroll
+    for (int s = T / 2; s >= Stor<O>::LPI; s >>= 1) {
+        if (s < lim) {
+            if (tid < s) {
+                Xyzz<O> a = sh[tid], b = sh[tid + s];
+                xyzz_add<O>(a, a, b);
+                sh[tid] = a;
+            }
+            __syncthreads();
+        }
+    }
+}
+
 // One workgroup per queued heavy bucket (grid-stride over the queue): strided
 // per-thread sums of its chunk partials, then a tree in LDS.
 template <class O, int THREADS>
@@ -371,18 +394,7 @@ __global__ void __launch_bounds__(THREADS) k_bucket_fixup_heavy(const MsmDesc* _
         }
         sh[tid] = acc;
         __syncthreads();
-        // tree over the items that hold parts (uniform over the block)
-        const uint32_t items = min((uint32_t)(THREADS / Stor<O>::LPI), parts);
-        uint32_t span = 1;
-        while (span < items) span <<= 1;
-        for (int s = (int)(span / 2) * Stor<O>::LPI; s >= Stor<O>::LPI; s >>= 1) {
-            if (tid < s) {
-                Xyzz<O> a = sh[tid], b = sh[tid + s];
-                xyzz_add<O>(a, a, b);
-                sh[tid] = a;
-            }
-            __syncthreads();
-        }
+        lds_tree<O, THREADS>(sh, tid, min((uint32_t)(THREADS / Stor<O>::LPI), parts));
         if (tid < Stor<O>::LPI) stp(&buckets[g], sh[tid]);
         __syncthreads();
     }
@@ -458,18 +470,7 @@ __global__ void __launch_bounds__(64 * Stor<O>::LPI) k_window_partial(const MsmD
     }
     sh[tid] = acc;
     __syncthreads();
-    // tree over the items that hold segments only (uniform over the block)
-    const uint32_t items = min((uint32_t)(T / Stor<O>::LPI), t1 - k * WSUM_TILE);
-    uint32_t span = 1;
-    while (span < items) span <<= 1;
-    for (int s = (int)(span / 2) * Stor<O>::LPI; s >= Stor<O>::LPI; s >>= 1) {
-        if (tid < s) {
-            Xyzz<O> a = sh[tid], b = sh[tid + s];
-            xyzz_add<O>(a, a, b);
-            sh[tid] = a;
-        }
-        __syncthreads();
-    }
+    lds_tree<O, T>(sh, tid, min((uint32_t)(T / Stor<O>::LPI), t1 - k * WSUM_TILE));
     if (tid < Stor<O>::LPI) stp(&tiles[gt], sh[tid]);
 }
 
@@ -494,17 +495,7 @@ __global__ void __launch_bounds__(THREADS) k_window_sum(const MsmDesc* __restric
     }
     sh[tid] = acc;
     __syncthreads();
-    const uint32_t items = min((uint32_t)(THREADS / Stor<O>::LPI), ntile);
-    uint32_t span = 1;
-    while (span < items) span <<= 1;
-    for (int s = (int)(span / 2) * Stor<O>::LPI; s >= Stor<O>::LPI; s >>= 1) {
-        if (tid < s) {
-            Xyzz<O> a = sh[tid], b = sh[tid + s];
-            xyzz_add<O>(a, a, b);
-            sh[tid] = a;
-        }
-        __syncthreads();
-    }
+    lds_tree<O, THREADS>(sh, tid, min((uint32_t)(THREADS / Stor<O>::LPI), ntile));
     if (tid < Stor<O>::LPI) stp(&wsum[gw], sh[tid]);
 }
 
