@@ -228,6 +228,11 @@ def test_full_size_closed_form(golden, curve, lg):
     s = gm.Scalars.upload(ss)
     r = gm.msm_resident(b, s, scalar_bits=299)
     assert as_point(curve, r[0]) == golden_point(curve, w["result"])
+    # the drop-in call with host buffers (copied through pinned staging by
+    # several threads) gives the same point
+    proj = gm.G1Projective if curve == "g1" else gm.G2Projective
+    got = proj.msm_multi(b.download(0, n), ss, montgomery=True, scalars_lt_r=True).to_affine().rec
+    assert same(got, r)
     off = n // 2 + 3
     sub = gm.msm_resident(b, s, n=1024, base_offset=off, scalar_offset=off, scalar_bits=299)
     assert same(sub, o.msm(curve, o.bases_from_k(curve, ks[off:off + 1024]), ss[off:off + 1024]))
