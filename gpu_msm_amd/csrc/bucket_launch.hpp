@@ -1,8 +1,11 @@
 // Launchers of the bucket kernels (accumulation, fix-up, bucket reduction,
 // window sums), shared by the per-curve translation units. O is the storage
 // policy; the kernels run with KernelOps<O> (Fp2LaneOps for G2: two lanes per
-// item, so every launch has Stor<K>::LPI threads per item).
+// item, so every launch has Stor<K>::LPI threads per item); a G1 plan with
+// tail_pair runs the bucket reduction and window sums with FpPairOps.
 #pragma once
+#include <type_traits>
+
 #include "engine_impl.hpp"
 #include "msm_kernels.hpp"
 
@@ -42,11 +45,9 @@ hipError_t launch_fixup(hipStream_t s, const Plan& p, uint8_t* ws) {
     return hipGetLastError();
 }
 
-template <class O>
-hipError_t launch_reduce(hipStream_t s, const Plan& p, uint8_t* ws) {
-    using K = typename KernelOps<O>::type;
+template <class K>
+hipError_t launch_reduce_k(hipStream_t s, const Plan& p, uint8_t* ws) {
     using X = typename Stor<K>::X;
-    if (p.seg_total == 0) return hipSuccess;
     k_bucket_reduce<K><<<blocks_for((long)p.seg_total * Stor<K>::LPI, 64), 64, 0, s>>>(
         reinterpret_cast<const MsmDesc*>(ws + p.o_desc), p.count, p.seg_total,
         reinterpret_cast<const X*>(ws + p.o_buckets), reinterpret_cast<X*>(ws + p.o_segs));
@@ -54,10 +55,16 @@ hipError_t launch_reduce(hipStream_t s, const Plan& p, uint8_t* ws) {
 }
 
 template <class O>
-hipError_t launch_window_sum(hipStream_t s, const Plan& p, uint8_t* ws) {
-    using K = typename KernelOps<O>::type;
+hipError_t launch_reduce(hipStream_t s, const Plan& p, uint8_t* ws) {
+    if (p.seg_total == 0) return hipSuccess;
+    if constexpr (std::is_same<O, FpOps>::value)
+        if (p.tail_pair) return launch_reduce_k<FpPairOps>(s, p, ws);
+    return launch_reduce_k<typename KernelOps<O>::type>(s, p, ws);
+}
+
+template <class K>
+hipError_t launch_window_sum_k(hipStream_t s, const Plan& p, uint8_t* ws) {
     using X = typename Stor<K>::X;
-    if (p.win_total == 0) return hipSuccess;
     const MsmDesc* desc = reinterpret_cast<const MsmDesc*>(ws + p.o_desc);
     // windows with few segments: one lane each (MSMs flagged DESC_WSUM_DIRECT)
     k_window_direct<K><<<blocks_for((long)p.win_total * Stor<K>::LPI, 64), 64, 0, s>>>(
@@ -71,6 +78,14 @@ hipError_t launch_window_sum(hipStream_t s, const Plan& p, uint8_t* ws) {
     k_window_sum<K, WT><<<p.win_total, WT, 0, s>>>(desc, p.count, reinterpret_cast<const X*>(ws + p.o_tiles),
                                                    reinterpret_cast<X*>(ws + p.o_wsum));
     return hipGetLastError();
+}
+
+template <class O>
+hipError_t launch_window_sum(hipStream_t s, const Plan& p, uint8_t* ws) {
+    if (p.win_total == 0) return hipSuccess;
+    if constexpr (std::is_same<O, FpOps>::value)
+        if (p.tail_pair) return launch_window_sum_k<FpPairOps>(s, p, ws);
+    return launch_window_sum_k<typename KernelOps<O>::type>(s, p, ws);
 }
 
 }  // namespace gm

@@ -829,4 +829,30 @@ struct Fp2LaneOps {
     }
 };
 
+
+// G1 policy of the depth-bound tail kernels (bucket reduction, window sums):
+// lanes 2k and 2k+1 both hold the full operands of one point operation and
+// split its independent field products between them (part 0 and part 1 of
+// each step), swapping results with the same DPP quad permutation as
+// Fp2LaneOps. The field operations are FpOps'; the point formulas are
+// specialised for this policy (xyzz_add / xyzz_dbl), and every branch depends
+// on values both lanes hold, so control flow is uniform over each pair.
+struct FpPairOps : FpOps {};
+
+// r = c ? a : b, limb by limb.
+__device__ __forceinline__ void fp_sel(Fp& r, bool c, const Fp& a, const Fp& b) {
+#pragma unroll
+    for (int i = 0; i < NL; ++i) r.v[i] = c ? a.v[i] : b.v[i];
+}
+
+// After a paired step in which part 0 computed r0 and part 1 computed r1
+// (each lane holding its own result in `mine`), gives both lanes both.
+__device__ __forceinline__ void pair_results(Fp& r0, Fp& r1, const Fp& mine) {
+    Fp other;
+    fp_pair_swap(other, mine);
+    const bool q = lane_part() != 0;
+    fp_sel(r0, q, other, mine);
+    fp_sel(r1, q, mine, other);
+}
+
 }  // namespace gm

@@ -23,6 +23,7 @@
 #pragma once
 #include "curve.hpp"
 #include "desc_find.hpp"
+#include "pair_curve.hpp"
 
 namespace gm {
 
@@ -119,6 +120,13 @@ struct Stor<Fp2LaneOps> {
     using X = Xyzz<Fp2Ops>;
     static constexpr int LPI = 2;
 };
+// G1 tail kernels: two lanes per item, each holding the whole record.
+template <>
+struct Stor<FpPairOps> {
+    using A = Aff<FpOps>;
+    using X = Xyzz<FpOps>;
+    static constexpr int LPI = 2;
+};
 
 template <class O>
 __device__ __forceinline__ void ldp(Aff<O>& d, const Aff<O>* s) {
@@ -143,6 +151,13 @@ __device__ __forceinline__ void ldp(Xyzz<Fp2LaneOps>& d, const Xyzz<Fp2Ops>* s) 
     ld16(d.Y, &s->Y.c0 + q);
     ld16(d.ZZ, &s->ZZ.c0 + q);
     ld16(d.ZZZ, &s->ZZZ.c0 + q);
+}
+__device__ __forceinline__ void ldp(Xyzz<FpPairOps>& d, const Xyzz<FpOps>* s) {
+    ld16(d, reinterpret_cast<const Xyzz<FpPairOps>*>(s));
+}
+// Both lanes of a pair hold the same record; the first stores it.
+__device__ __forceinline__ void stp(Xyzz<FpOps>* d, const Xyzz<FpPairOps>& v) {
+    if (lane_part() == 0) st16(reinterpret_cast<Xyzz<FpPairOps>*>(d), v);
 }
 __device__ __forceinline__ void stp(Xyzz<Fp2Ops>* d, const Xyzz<Fp2LaneOps>& v) {
     const uint32_t q = lane_part();

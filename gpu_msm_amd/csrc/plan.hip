@@ -72,15 +72,18 @@ bool make_plan(Plan& p, MsmDesc* d, int curve, int L, int bits, const MsmSpec* s
     p.host_desc = d;
     // Buckets per bucket-reduction lane for the MSMs that leave it to the
     // plan: the smallest count >= 4 that keeps the batch's reduction within
-    // REDUCE_THREADS threads (G2 runs two threads per lane), i.e. about one
-    // wave per SIMD (the reduction runs at one wave per SIMD, so a second
-    // round would double its time, while fewer waves leave SIMDs idle and
-    // lengthen each lane's running sums), capped where a batch needs more
-    // than one round. Small MSMs get 4, the sweeps' optimum for them; at
-    // N = 2^20 G1 gets 6 and G2 11.
+    // REDUCE_THREADS threads, i.e. about one wave per SIMD (the reduction
+    // runs at one wave per SIMD, so a second round would double its time,
+    // while fewer waves leave SIMDs idle and lengthen each lane's running
+    // sums), capped where a batch needs more than one round (16 for G1, 32
+    // for G2). G2 runs two threads per segment (a lane pair). G1 runs two
+    // (a pair splitting each point operation's products) when the batch's
+    // segments then still fit one round, else one: past one round the
+    // reduction is bound by throughput, where the single-lane kernels do
+    // less work. Small MSMs get 4, the sweeps' optimum for them; at
+    // N = 2^20 both curves get 11.
     constexpr uint64_t REDUCE_THREADS = 62 * 1024;
-    const uint64_t lpi = curve == CURVE_G2 ? 2 : 1;
-    uint64_t auto_bkt = 0, fixed_thr = 0;
+    uint64_t auto_bkt = 0, fixed_segs = 0;
     for (int j = 0; j < count; ++j) {
         const MsmSpec& s = specs[j];
         const int c = s.c > 0 ? s.c : choose_window(curve, s.n, bits);
@@ -88,17 +91,26 @@ bool make_plan(Plan& p, MsmDesc* d, int curve, int L, int bits, const MsmSpec* s
         const uint64_t B = 1ull << (c - 1), nbk = B * (uint64_t)windows_for(bits, c);
         if (s.seg > 0) {
             const uint64_t sg = std::min<uint64_t>(B, (uint64_t)s.seg);
-            fixed_thr += (B + sg - 1) / sg * windows_for(bits, c) * lpi;
+            fixed_segs += (B + sg - 1) / sg * windows_for(bits, c);
         } else {
             auto_bkt += nbk;
         }
     }
-    uint64_t auto_seg = 4;
-    if (fixed_thr < REDUCE_THREADS)
-        auto_seg = std::max<uint64_t>(auto_seg, (auto_bkt * lpi + (REDUCE_THREADS - fixed_thr) - 1) /
-                                                    (REDUCE_THREADS - fixed_thr));
-    // beyond one round of waves the sweeps favour 16 for G1 and 32 for G2
-    auto_seg = std::min<uint64_t>(auto_seg, curve == CURVE_G2 ? 32 : 16);
+    const uint64_t seg_cap = curve == CURVE_G2 ? 32 : 16;
+    auto seg_for = [&](uint64_t lpi) {
+        uint64_t sg = 4;
+        if (fixed_segs * lpi < REDUCE_THREADS)
+            sg = std::max<uint64_t>(sg, (auto_bkt * lpi + (REDUCE_THREADS - fixed_segs * lpi) - 1) /
+                                            (REDUCE_THREADS - fixed_segs * lpi));
+        return sg;
+    };
+    uint64_t auto_seg = seg_for(2);
+    p.tail_pair = curve == CURVE_G1;
+    if (p.tail_pair && auto_seg > seg_cap) {
+        p.tail_pair = false;
+        auto_seg = seg_for(1);
+    }
+    auto_seg = std::min(auto_seg, seg_cap);
     uint64_t pt = 0, rank = 0, bkt = 0, seg = 0, win = 0, tile = 0;
     for (int j = 0; j < count; ++j) {
         const MsmSpec& s = specs[j];
@@ -162,6 +174,9 @@ bool make_plan(Plan& p, MsmDesc* d, int curve, int L, int bits, const MsmSpec* s
         }
         if (tb >= 0xFFFFFFFFull) return false;
     }
+    // paired G1 tail kernels only while the pairs of all segments fit one
+    // round of one wave per SIMD (1024 SIMDs x 64 lanes)
+    if (p.tail_pair && 2 * seg > 64 * 1024) p.tail_pair = false;
     p.stile_total = (uint32_t)stile;
     p.lds_B = lds_sort ? maxB : 0;
     p.n_total = (uint32_t)pt;

@@ -275,6 +275,24 @@ def test_batch_matches_alone_and_oracle(curve):
             assert same(got[k:k + 1], want), k
 
 
+def test_g1_tail_pair_and_single_lane_agree():
+    """The G1 bucket reduction and window sums run on lane pairs (each point
+    operation's products split over two lanes) while the plan's segments fit
+    one round of waves, and on single lanes past it. One MSM alone (25
+    windows x 512 segments of 4 buckets: 25,600 threads) takes the paired
+    kernels; the same MSM in a batch of 64 (819,200 segments) the single-lane
+    ones. Both equal the oracle."""
+    pool_n = 1 << 14
+    ks, b = _pool("g1", pool_n, 120)
+    ss = o.gen_scalars_modr(121, pool_n)
+    s = gm.Scalars.upload(ss)
+    alone = gm.msm_batch(b, s, [(pool_n, 0, 0, 12, 4)], scalar_bits=299)
+    many = gm.msm_batch(b, s, [(pool_n, 0, 0, 12, 4)] * 64, scalar_bits=299)
+    for k in (0, 31, 63):
+        assert same(many[k:k + 1], alone), k
+    assert same(alone, o.msm("g1", o.bases_from_k("g1", ks), ss))
+
+
 @pytest.mark.parametrize("curve", ["g1", "g2"])
 def test_batch_many_windows_direct_window_sums(curve):
     """A batch of over 1024 windows sums windows of few segments with one
