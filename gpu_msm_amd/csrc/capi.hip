@@ -149,6 +149,15 @@ bool ensure_staging(DevCtx* c) {
     for (int w = 0; w < Staging::WORKERS; ++w)
         for (int k = 0; k < Staging::SLOTS; ++k)
             if (hipEventCreateWithFlags(&st.ev[w][k], hipEventDisableTiming) != hipSuccess) {
+                st.ev[w][k] = nullptr;
+                for (auto& row : st.ev)
+                    for (hipEvent_t& e : row)
+                        if (e) {
+                            hipEventDestroy(e);
+                            e = nullptr;
+                        }
+                hipHostFree(st.buf);
+                st.buf = nullptr;
                 st.failed = true;
                 return false;
             }
