@@ -275,6 +275,39 @@ def test_batch_matches_alone_and_oracle(curve):
             assert same(got[k:k + 1], want), k
 
 
+@pytest.mark.parametrize("curve", ["g1", "g2"])
+def test_reduction_and_window_sum_special_cases(curve):
+    """Equal and opposite points meet inside the bucket reduction's running
+    sums and between a window's segment results, so the additions there take
+    their doubling and infinity branches (G1 on lane pairs, G2 on Fp2 lane
+    pairs). Window 4 (8 buckets, two segments of 4), scalars below 16, points
+    Q, 5Q, 4Q and -Q; each case against the oracle."""
+    cv = gm.CURVE_G1 if curve == "g1" else gm.CURVE_G2
+    k0 = 987654321
+    base = o.bases_from_k(curve, np.array([k0, 5 * k0, 4 * k0], dtype=np.uint64))
+    neg = base[0:1].copy()
+    y = neg[0]["y"]
+    for h in range(len(y) // 7):  # -y, component by component
+        y[7 * h:7 * h + 7] = o.fp_unop("or_fp_neg", y[7 * h:7 * h + 7].copy())
+    pts = np.concatenate([base, neg])  # Q, 5Q, 4Q, -Q
+    cases = [
+        ([0, 0], [2, 1]),  # S = Q + Q in the running sum (doubling)
+        ([0], [2]),        # T = Q + Q after an empty bucket (doubling)
+        ([0, 3], [2, 1]),  # S = Q + (-Q) (infinity), result Q
+        ([1, 2], [4, 5]),  # segment results 20Q and 20Q (doubling in the window sum)
+        ([1, 2], [4, 5 + 16]),  # the same in the low window, 16 * 4Q in the next
+    ]
+    for idx, sc in cases:
+        sel = pts[idx]
+        scal = np.array([[v, 0, 0, 0, 0] for v in sc], dtype=np.uint64)
+        s = gm.Scalars.upload(scal)
+        bb = gm.Bases.upload(cv, sel)
+        got = gm.msm_batch(bb, s, [(len(idx), 0, 0, 4, 4)], scalar_bits=299)
+        assert same(got, o.msm(curve, sel, scal)), (idx, sc)
+        bb.free()
+        s.free()
+
+
 def test_g1_tail_pair_and_single_lane_agree():
     """The G1 bucket reduction and window sums run on lane pairs (each point
     operation's products split over two lanes) while the plan's segments fit
