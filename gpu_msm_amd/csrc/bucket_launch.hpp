@@ -1,15 +1,26 @@
 // Launchers of the bucket kernels (accumulation, fix-up, bucket reduction,
 // window sums), shared by the per-curve translation units. O is the storage
 // policy; the kernels run with KernelOps<O> (Fp2LaneOps for G2: two lanes per
-// item, so every launch has Stor<K>::LPI threads per item); a G1 plan with
-// tail_pair runs the bucket reduction and window sums with FpPairOps.
+// item, so every launch has Stor<K>::LPI threads per item); a plan with
+// tail_split runs the bucket reduction and window sums with SplitOps<O>
+// (FpPairOps for G1, Fp2QuadOps for G2).
 #pragma once
-#include <type_traits>
-
 #include "engine_impl.hpp"
 #include "msm_kernels.hpp"
 
 namespace gm {
+
+// Split lane groups of the tail kernels per storage policy.
+template <class O>
+struct SplitOps;
+template <>
+struct SplitOps<FpOps> {
+    using type = FpPairOps;
+};
+template <>
+struct SplitOps<Fp2Ops> {
+    using type = Fp2QuadOps;
+};
 
 template <class O>
 hipError_t launch_accumulate(hipStream_t s, const Plan& p, const Aff<O>* d_bases, uint8_t* ws) {
@@ -57,8 +68,7 @@ hipError_t launch_reduce_k(hipStream_t s, const Plan& p, uint8_t* ws) {
 template <class O>
 hipError_t launch_reduce(hipStream_t s, const Plan& p, uint8_t* ws) {
     if (p.seg_total == 0) return hipSuccess;
-    if constexpr (std::is_same<O, FpOps>::value)
-        if (p.tail_pair) return launch_reduce_k<FpPairOps>(s, p, ws);
+    if (p.tail_split) return launch_reduce_k<typename SplitOps<O>::type>(s, p, ws);
     return launch_reduce_k<typename KernelOps<O>::type>(s, p, ws);
 }
 
@@ -83,8 +93,7 @@ hipError_t launch_window_sum_k(hipStream_t s, const Plan& p, uint8_t* ws) {
 template <class O>
 hipError_t launch_window_sum(hipStream_t s, const Plan& p, uint8_t* ws) {
     if (p.win_total == 0) return hipSuccess;
-    if constexpr (std::is_same<O, FpOps>::value)
-        if (p.tail_pair) return launch_window_sum_k<FpPairOps>(s, p, ws);
+    if (p.tail_split) return launch_window_sum_k<typename SplitOps<O>::type>(s, p, ws);
     return launch_window_sum_k<typename KernelOps<O>::type>(s, p, ws);
 }
 

@@ -100,7 +100,7 @@ struct Plan {
     uint32_t stile_total;  // sort tiles (0: the global-atomic fallback sort)
     uint32_t lds_B;        // largest bucket count of the batch (LDS counters per sort tile)
     int lanes;      // accumulation lanes (upper bound: all digits non-zero)
-    bool tail_pair; // G1 bucket reduction and window sums on lane pairs (FpPairOps)
+    bool tail_split; // bucket reduction and window sums on split lane groups (FpPairOps / Fp2QuadOps)
     size_t cub_bytes;
     // workspace offsets (bytes)
     size_t o_desc, o_cnt, o_off, o_rank, o_dig, o_tbase, o_sorted, o_buckets, o_head, o_tail, o_heavy, o_segs, o_tiles, o_wsum, o_out, o_err,

@@ -855,4 +855,29 @@ __device__ __forceinline__ void pair_results(Fp& r0, Fp& r1, const Fp& mine) {
     fp_sel(r1, q, mine, other);
 }
 
+// G2 policy of the tail kernels: lanes 4k..4k+3 form two Fp2LaneOps pairs
+// (lanes 4k, 4k+1 and 4k+2, 4k+3) that hold the same operands and compute
+// the two independent Fp2 products of each step of a point operation, one
+// per pair (half 0 and half 1), exchanging results with the DPP quad
+// permutation [2, 3, 0, 1].
+struct Fp2QuadOps : Fp2LaneOps {};
+
+__device__ __forceinline__ uint32_t quad_half() { return (threadIdx.x >> 1) & 1u; }
+
+// r = the value of a in the other pair of the quad (quad_perm [2, 3, 0, 1]).
+__device__ __forceinline__ void fp_half_swap(Fp& r, const Fp& a) {
+#pragma unroll
+    for (int i = 0; i < NL; ++i) r.v[i] = (uint32_t)__builtin_amdgcn_mov_dpp((int)a.v[i], 0x4E, 0xF, 0xF, false);
+}
+
+// After a step in which half 0 computed r0 and half 1 computed r1, gives
+// both halves both (each lane its own Fp2 component).
+__device__ __forceinline__ void quad_results(Fp& r0, Fp& r1, const Fp& mine) {
+    Fp other;
+    fp_half_swap(other, mine);
+    const bool h = quad_half() != 0;
+    fp_sel(r0, h, other, mine);
+    fp_sel(r1, h, mine, other);
+}
+
 }  // namespace gm

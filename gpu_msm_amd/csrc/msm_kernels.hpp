@@ -120,12 +120,19 @@ struct Stor<Fp2LaneOps> {
     using X = Xyzz<Fp2Ops>;
     static constexpr int LPI = 2;
 };
-// G1 tail kernels: two lanes per item, each holding the whole record.
+// Tail kernels on split lane groups: G1 two lanes per item, each holding the
+// whole record; G2 four (two lane pairs), each pair holding the record.
 template <>
 struct Stor<FpPairOps> {
     using A = Aff<FpOps>;
     using X = Xyzz<FpOps>;
     static constexpr int LPI = 2;
+};
+template <>
+struct Stor<Fp2QuadOps> {
+    using A = Aff<Fp2Ops>;
+    using X = Xyzz<Fp2Ops>;
+    static constexpr int LPI = 4;
 };
 
 template <class O>
@@ -158,6 +165,22 @@ __device__ __forceinline__ void ldp(Xyzz<FpPairOps>& d, const Xyzz<FpOps>* s) {
 // Both lanes of a pair hold the same record; the first stores it.
 __device__ __forceinline__ void stp(Xyzz<FpOps>* d, const Xyzz<FpPairOps>& v) {
     if (lane_part() == 0) st16(reinterpret_cast<Xyzz<FpPairOps>*>(d), v);
+}
+__device__ __forceinline__ void ldp(Xyzz<Fp2QuadOps>& d, const Xyzz<Fp2Ops>* s) {
+    const uint32_t q = lane_part();
+    ld16(d.X, &s->X.c0 + q);
+    ld16(d.Y, &s->Y.c0 + q);
+    ld16(d.ZZ, &s->ZZ.c0 + q);
+    ld16(d.ZZZ, &s->ZZZ.c0 + q);
+}
+// Both pairs of a quad hold the same record; the first stores it.
+__device__ __forceinline__ void stp(Xyzz<Fp2Ops>* d, const Xyzz<Fp2QuadOps>& v) {
+    if (quad_half() != 0) return;
+    const uint32_t q = lane_part();
+    st16(&d->X.c0 + q, v.X);
+    st16(&d->Y.c0 + q, v.Y);
+    st16(&d->ZZ.c0 + q, v.ZZ);
+    st16(&d->ZZZ.c0 + q, v.ZZZ);
 }
 __device__ __forceinline__ void stp(Xyzz<Fp2Ops>* d, const Xyzz<Fp2LaneOps>& v) {
     const uint32_t q = lane_part();

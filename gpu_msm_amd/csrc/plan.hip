@@ -76,12 +76,12 @@ bool make_plan(Plan& p, MsmDesc* d, int curve, int L, int bits, const MsmSpec* s
     // runs at one wave per SIMD, so a second round would double its time,
     // while fewer waves leave SIMDs idle and lengthen each lane's running
     // sums), capped where a batch needs more than one round (16 for G1, 32
-    // for G2). G2 runs two threads per segment (a lane pair). G1 runs two
-    // (a pair splitting each point operation's products) when the batch's
-    // segments then still fit one round, else one: past one round the
-    // reduction is bound by throughput, where the single-lane kernels do
-    // less work. Small MSMs get 4, the sweeps' optimum for them; at
-    // N = 2^20 both curves get 11.
+    // for G2). A segment takes one thread (G1) or a lane pair (G2), doubled
+    // when the tail kernels split each point operation's products over two
+    // such groups (tail_split): the plan splits while the batch's segments
+    // then still fit one round, and not past it, where the reduction is
+    // bound by throughput and the unsplit kernels do less work. Small MSMs
+    // get 4, the sweeps' optimum for them; at N = 2^20 G1 gets 11 and G2 21.
     constexpr uint64_t REDUCE_THREADS = 62 * 1024;
     uint64_t auto_bkt = 0, fixed_segs = 0;
     for (int j = 0; j < count; ++j) {
@@ -104,11 +104,12 @@ bool make_plan(Plan& p, MsmDesc* d, int curve, int L, int bits, const MsmSpec* s
                                             (REDUCE_THREADS - fixed_segs * lpi));
         return sg;
     };
-    uint64_t auto_seg = seg_for(2);
-    p.tail_pair = curve == CURVE_G1;
-    if (p.tail_pair && auto_seg > seg_cap) {
-        p.tail_pair = false;
-        auto_seg = seg_for(1);
+    const uint64_t group = curve == CURVE_G2 ? 2 : 1;  // threads per segment, unsplit
+    uint64_t auto_seg = seg_for(2 * group);
+    p.tail_split = true;
+    if (auto_seg > seg_cap) {
+        p.tail_split = false;
+        auto_seg = seg_for(group);
     }
     auto_seg = std::min(auto_seg, seg_cap);
     uint64_t pt = 0, rank = 0, bkt = 0, seg = 0, win = 0, tile = 0;
@@ -174,9 +175,9 @@ bool make_plan(Plan& p, MsmDesc* d, int curve, int L, int bits, const MsmSpec* s
         }
         if (tb >= 0xFFFFFFFFull) return false;
     }
-    // paired G1 tail kernels only while the pairs of all segments fit one
-    // round of one wave per SIMD (1024 SIMDs x 64 lanes)
-    if (p.tail_pair && 2 * seg > 64 * 1024) p.tail_pair = false;
+    // split tail kernels only while all segments' lane groups fit one round
+    // of one wave per SIMD (1024 SIMDs x 64 lanes)
+    if (p.tail_split && 2 * group * seg > 64 * 1024) p.tail_split = false;
     p.stile_total = (uint32_t)stile;
     p.lds_B = lds_sort ? maxB : 0;
     p.n_total = (uint32_t)pt;

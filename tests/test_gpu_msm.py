@@ -279,8 +279,9 @@ def test_batch_matches_alone_and_oracle(curve):
 def test_reduction_and_window_sum_special_cases(curve):
     """Equal and opposite points meet inside the bucket reduction's running
     sums and between a window's segment results, so the additions there take
-    their doubling and infinity branches (G1 on lane pairs, G2 on Fp2 lane
-    pairs). Window 4 (8 buckets, two segments of 4), scalars below 16, points
+    their doubling and infinity branches (on the split lane groups: G1 lane
+    pairs, G2 pairs of Fp2 lane pairs). Window 4 (8 buckets, two segments of
+    4), scalars below 16, points
     Q, 5Q, 4Q and -Q; each case against the oracle."""
     cv = gm.CURVE_G1 if curve == "g1" else gm.CURVE_G2
     k0 = 987654321
@@ -308,22 +309,23 @@ def test_reduction_and_window_sum_special_cases(curve):
         s.free()
 
 
-def test_g1_tail_pair_and_single_lane_agree():
-    """The G1 bucket reduction and window sums run on lane pairs (each point
-    operation's products split over two lanes) while the plan's segments fit
-    one round of waves, and on single lanes past it. One MSM alone (25
-    windows x 512 segments of 4 buckets: 25,600 threads) takes the paired
-    kernels; the same MSM in a batch of 64 (819,200 segments) the single-lane
-    ones. Both equal the oracle."""
+@pytest.mark.parametrize("curve", ["g1", "g2"])
+def test_tail_split_and_unsplit_agree(curve):
+    """The bucket reduction and window sums split each point operation's
+    products over two lane groups (G1: two lanes; G2: two lane pairs) while
+    the plan's segments fit one round of waves, and run unsplit past it. One
+    MSM alone (25 windows x 512 segments of 4 buckets: 25,600 G1 or 51,200
+    G2 threads) takes the split kernels; the same MSM in a batch of 64
+    (819,200 segments) the unsplit ones. Both equal the oracle."""
     pool_n = 1 << 14
-    ks, b = _pool("g1", pool_n, 120)
+    ks, b = _pool(curve, pool_n, 120)
     ss = o.gen_scalars_modr(121, pool_n)
     s = gm.Scalars.upload(ss)
     alone = gm.msm_batch(b, s, [(pool_n, 0, 0, 12, 4)], scalar_bits=299)
     many = gm.msm_batch(b, s, [(pool_n, 0, 0, 12, 4)] * 64, scalar_bits=299)
     for k in (0, 31, 63):
         assert same(many[k:k + 1], alone), k
-    assert same(alone, o.msm("g1", o.bases_from_k("g1", ks), ss))
+    assert same(alone, o.msm(curve, o.bases_from_k(curve, ks), ss))
 
 
 @pytest.mark.parametrize("curve", ["g1", "g2"])
