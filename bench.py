@@ -65,19 +65,25 @@ def measured_traffic() -> float | None:
 
 
 def time_msm(bases, scalars, steps: int, warmup: int, profile: bool):
-    # profiling (per-stage HIP events) is switched on before the warm-up, so
-    # the first timed steps do not pay for the events' first use
-    gm.set_profiling(profile)
+    """Mean wall time of `steps` MSMs after `warmup`, without instrumentation;
+    then, with `profile`, the per-stage HIP-event times of as many more MSMs
+    (events on the MSM's own stream, switched on one MSM ahead so their first
+    use is not counted)."""
+    gm.set_profiling(False)
     for _ in range(warmup):
         res = gm.msm_resident(bases, scalars, scalar_bits=299)
-    stages = []
     t0 = time.perf_counter()
     for _ in range(steps):
         res = gm.msm_resident(bases, scalars, scalar_bits=299)
-        if profile:
-            stages.append(gm.profile_last())
     dt = time.perf_counter() - t0
-    gm.set_profiling(False)
+    stages = []
+    if profile:
+        gm.set_profiling(True)
+        gm.msm_resident(bases, scalars, scalar_bits=299)
+        for _ in range(steps):
+            gm.msm_resident(bases, scalars, scalar_bits=299)
+            stages.append(gm.profile_last())
+        gm.set_profiling(False)
     return res, dt * 1e3 / steps, stages
 
 
