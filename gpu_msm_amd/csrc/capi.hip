@@ -215,8 +215,14 @@ hipError_t h2d_staged(DevCtx* c, const H2D* cp, int ncp) {
         err[w] = e;
     };
     std::vector<std::thread> th;
-    th.reserve(workers - 1);
-    for (int w = 1; w < workers; ++w) th.emplace_back(run, w);
+    int started = 1;
+    try {
+        th.reserve(workers - 1);
+        for (; started < workers; ++started) th.emplace_back(run, started);
+    } catch (...) {
+        // a thread that cannot start leaves its part (and the later ones) to this thread
+    }
+    for (int w = started; w < workers; ++w) run(w);
     run(0);
     for (auto& t : th) t.join();
     for (hipError_t e : err)

@@ -59,6 +59,30 @@ def test_g1_msm_wrapper_u64_vs_oracle():
     assert same(got, want)
 
 
+@pytest.mark.parametrize("curve,n", [("g1", 70001), ("g1", 300007), ("g2", 40009)])
+def test_host_buffer_staging_sizes(curve, n):
+    """Host-buffer calls of 8 MiB and more go through pinned staging slots
+    split over worker threads: sizes that fill two workers, eight workers
+    with a short last slot, and G2 records. The result equals the same MSM
+    over resident copies of the same data."""
+    cv = gm.CURVE_G1 if curve == "g1" else gm.CURVE_G2
+    ks = o.gen_u64(130, n)
+    ss = o.gen_scalars_modr(131, n)
+    b = gm.Bases.from_k(cv, ks)
+    s = gm.Scalars.upload(ss)
+    ref = gm.msm_resident(b, s, scalar_bits=299)
+    proj = gm.G1Projective if curve == "g1" else gm.G2Projective
+    got = proj.msm_multi(b.download(0, n), ss, montgomery=True, scalars_lt_r=True).to_affine().rec
+    assert same(got, ref)
+    # uploads through the same staging: the bases and scalars round-trip
+    b2 = gm.Bases.upload(cv, b.download(0, n))
+    assert same(b2.download(0, n), b.download(0, n))
+    s2 = gm.Scalars.upload(ss)
+    assert same(gm.msm_resident(b2, s2, scalar_bits=299), ref)
+    for h in (b, s, b2, s2):
+        h.free()
+
+
 def test_g2_msm_wrapper_u64_vs_oracle():
     n = 100
     ks = o.gen_u64(79, n)
