@@ -125,6 +125,8 @@ __global__ void __launch_bounds__(256) k_scatter(const MsmDesc* __restrict__ des
 //               its final position.
 // Global atomics drop from one per entry to one per (tile, bucket).
 constexpr uint16_t DIG_ZERO = 0xFFFF;
+// digits loaded ahead per thread in the tile kernels
+constexpr int SORT_UNROLL = 8;
 
 __global__ void __launch_bounds__(256) k_digits(const MsmDesc* __restrict__ desc, int count, uint32_t n_total,
                                                 const uint64_t* __restrict__ scalars, int L,
@@ -169,9 +171,18 @@ __global__ void __launch_bounds__(1024) k_tile_count(const MsmDesc* __restrict__
     for (uint32_t b = threadIdx.x; b < m.B; b += blockDim.x) lc[b] = 0;
     __syncthreads();
     const uint16_t* dw = dig + (size_t)m.rank_base + (size_t)t.w * m.n;
-    for (uint32_t i = t.i0 + threadIdx.x; i < t.i1; i += blockDim.x) {
-        const uint32_t code = dw[i];
-        if (code != DIG_ZERO) atomicAdd(&lc[code & 0x7FFFu], 1u);
+    // SORT_UNROLL independent digit loads in flight per thread before their
+    // use: one memory latency per SORT_UNROLL entries, not per entry
+    for (uint32_t i = t.i0 + threadIdx.x; i < t.i1; i += SORT_UNROLL * blockDim.x) {
+        uint32_t code[SORT_UNROLL];
+#pragma unroll
+        for (int u = 0; u < SORT_UNROLL; ++u) {
+            const uint32_t j = i + u * blockDim.x;
+            code[u] = j < t.i1 ? dw[j] : DIG_ZERO;
+        }
+#pragma unroll
+        for (int u = 0; u < SORT_UNROLL; ++u)
+            if (code[u] != DIG_ZERO) atomicAdd(&lc[code[u] & 0x7FFFu], 1u);
     }
     __syncthreads();
     uint32_t* tb = tbase + m.tb_base + (size_t)t.local * m.B;
@@ -196,13 +207,21 @@ __global__ void __launch_bounds__(1024) k_tile_place(const MsmDesc* __restrict__
     for (uint32_t b = threadIdx.x; b < m.B; b += blockDim.x) pos[b] = off[g0 + b] + tb[b];
     __syncthreads();
     const uint16_t* dw = dig + (size_t)m.rank_base + (size_t)t.w * m.n;
-    for (uint32_t i = t.i0 + threadIdx.x; i < t.i1; i += blockDim.x) {
-        const uint32_t code = dw[i];
-        if (code == DIG_ZERO) continue;
-        const uint32_t b = code & 0x7FFFu;
-        const uint32_t at = atomicAdd(&pos[b], 1u);
-        // (global bucket << 32) | base index in the pool | sign bit
-        sorted[at] = ((uint64_t)(g0 + b) << 32) | (m.base_off + i) | ((code & 0x8000u) << 16);
+    for (uint32_t i = t.i0 + threadIdx.x; i < t.i1; i += SORT_UNROLL * blockDim.x) {
+        uint32_t code[SORT_UNROLL];
+#pragma unroll
+        for (int u = 0; u < SORT_UNROLL; ++u) {
+            const uint32_t j = i + u * blockDim.x;
+            code[u] = j < t.i1 ? dw[j] : DIG_ZERO;
+        }
+#pragma unroll
+        for (int u = 0; u < SORT_UNROLL; ++u) {
+            if (code[u] == DIG_ZERO) continue;
+            const uint32_t b = code[u] & 0x7FFFu;
+            const uint32_t at = atomicAdd(&pos[b], 1u);
+            // (global bucket << 32) | base index in the pool | sign bit
+            sorted[at] = ((uint64_t)(g0 + b) << 32) | (m.base_off + i + u * blockDim.x) | ((code[u] & 0x8000u) << 16);
+        }
     }
 }
 
