@@ -200,10 +200,12 @@ def bench_served(requests: int) -> dict:
     return out
 
 
-def cpu_baseline(bases_dev, scalars_np: np.ndarray, curve: str, n_full: int):
+def cpu_baseline(bases_dev, ks: np.ndarray, scalars_np: np.ndarray, curve: str, n_full: int):
     """Oracle host Pippenger on a bounded sample (N = 2^16, window 16) on all
-    host cores, scaled to N = 2^20 by the window-16 cost model
-    (N + 2 * 2^16 projective adds per window)."""
+    host cores (at most 16), scaled to N = 2^20 by the window-16 cost model
+    (N + 2 * 2^16 projective adds per window); the same sample on one core;
+    and, measured directly, the N = 1024 host configuration on one core and a
+    G2 MSM at N = 2^14 on all cores."""
     from oracle import oracle as o
 
     o.build()
@@ -213,14 +215,31 @@ def cpu_baseline(bases_dev, scalars_np: np.ndarray, curve: str, n_full: int):
     t0 = time.perf_counter()
     o.msm(curve, pts, scalars_np[:ns], c=16, threads=threads)
     dt = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    o.msm(curve, pts, scalars_np[:ns], c=16, threads=1)
+    dt1 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    o.msm("g1", pts[:1024], scalars_np[:1024], threads=1)
+    dt_c1 = time.perf_counter() - t0
+    n2 = 1 << 14
+    g2 = o.bases_from_k("g2", ks[:n2], threads=threads)
+    t0 = time.perf_counter()
+    o.msm("g2", g2, scalars_np[:n2], threads=threads)
+    dt_g2 = time.perf_counter() - t0
     scale = (n_full + 2 * 65536) / (ns + 2 * 65536)
+    lg = int(np.log2(n_full))
     return {
         "value": round(dt * 1e3 * scale, 1),
         "unit": "ms",
         "cores": threads,
         "kind": "port",
         "sample": f"oracle Pippenger, {curve.upper()} N=2^16 window 16, {dt:.2f} s measured, scaled x{scale:.2f} "
-                  f"to N=2^{int(np.log2(n_full))}",
+                  f"to N=2^{lg}",
+        "single_core": {"value": round(dt1 * 1e3 * scale, 1), "unit": "ms", "cores": 1,
+                        "sample": f"the same N=2^16 sample on one core, {dt1:.2f} s measured, scaled x{scale:.2f} "
+                                  f"to N=2^{lg}"},
+        "measured_directly": {"g1_n1024_one_core_ms": round(dt_c1 * 1e3, 2),
+                              "g2_n2p14_ms": round(dt_g2 * 1e3, 1), "g2_cores": threads},
     }
 
 
@@ -297,7 +316,7 @@ def main() -> None:
     cpu = None
     if not args.no_cpu_baseline:
         print("bench: cpu baseline (oracle, bounded sample)", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(b1, ss, "g1", n)
+        cpu = cpu_baseline(b1, ks, ss, "g1", n)
 
     out = {
         "metric": "G1/G2 MSM ms at N=2^20 (bit-exact); batched MSMs/s vs plain; p99 latency",
