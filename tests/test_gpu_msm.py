@@ -353,6 +353,34 @@ def test_tail_split_and_unsplit_agree(curve):
 
 
 @pytest.mark.parametrize("curve", ["g1", "g2"])
+def test_random_plans_against_oracle(curve):
+    """Seeded random MSMs (sizes 1..3000, windows 2..13, bucket segments of
+    any length or the plan's own), in one batch and each alone, against the
+    oracle: small windows and short segments reach the direct window sums, a
+    window of one segment and the last short segment."""
+    rng = np.random.default_rng(140 if curve == "g1" else 141)
+    pool_n = 4096
+    ks, b = _pool(curve, pool_n, 142)
+    ss = o.gen_scalars_modr(143, pool_n)
+    s = gm.Scalars.upload(ss)
+    specs = []
+    for _ in range(8):
+        n = int(rng.integers(1, 3001))
+        bo = int(rng.integers(0, pool_n - n + 1))
+        so = int(rng.integers(0, pool_n - n + 1))
+        c = int(rng.integers(2, 14))
+        seg = int(rng.choice([0, 1, 2, 3, 5, 7, 9, 16, 40]))
+        specs.append((n, bo, so, c, seg))
+    got = gm.msm_batch(b, s, specs, scalar_bits=299)
+    for k, (n, bo, so, c, seg) in enumerate(specs):
+        want = o.msm(curve, o.bases_from_k(curve, ks[bo:bo + n]), ss[so:so + n])
+        assert same(got[k:k + 1], want), specs[k]
+        alone = gm.msm_batch(b, s, [specs[k]], scalar_bits=299)
+        assert same(alone, want), specs[k]
+    s.free()
+
+
+@pytest.mark.parametrize("curve", ["g1", "g2"])
 def test_batch_many_windows_direct_window_sums(curve):
     """A batch of over 1024 windows sums windows of few segments with one
     lane each, and the rest with tiles; every MSM equals the same MSM run
