@@ -99,11 +99,13 @@ struct Plan {
     uint32_t n_total, ent_total, bkt_total, seg_total, win_total, tile_total;
     uint32_t stile_total;  // sort tiles (0: the global-atomic fallback sort)
     uint32_t lds_B;        // largest bucket count of the batch (LDS counters per sort tile)
+    uint32_t qmax;         // two-pass placement: block slots per sort tile (0: one pass)
     int lanes;      // accumulation lanes (upper bound: all digits non-zero)
     bool tail_split; // bucket reduction and window sums on split lane groups (FpPairOps / Fp2QuadOps)
     size_t cub_bytes;
     // workspace offsets (bytes)
-    size_t o_desc, o_cnt, o_off, o_rank, o_dig, o_tbase, o_sorted, o_buckets, o_head, o_tail, o_heavy, o_segs, o_tiles, o_wsum, o_out, o_err,
+    size_t o_desc, o_cnt, o_off, o_rank, o_dig, o_tbase, o_blk, o_qbase, o_tmp, o_sorted, o_buckets, o_head, o_tail,
+        o_heavy, o_segs, o_tiles, o_wsum, o_out, o_err,
         o_cub,
         total;
     MsmDesc* host_desc;  // owned by the caller of make_plan (count entries)

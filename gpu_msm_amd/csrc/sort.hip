@@ -33,6 +33,11 @@ hipError_t launch_sort(hipStream_t s, const Plan& p, const uint64_t* d_scalars, 
     const size_t lds = (size_t)p.lds_B * 4;
     uint16_t* dig = reinterpret_cast<uint16_t*>(ws + p.o_dig);
     uint32_t* tbase = reinterpret_cast<uint32_t*>(ws + p.o_tbase);
+    uint32_t* blkcnt = reinterpret_cast<uint32_t*>(ws + p.o_blk);
+    uint32_t* qbase = reinterpret_cast<uint32_t*>(ws + p.o_qbase);
+    uint64_t* tmp = reinterpret_cast<uint64_t*>(ws + p.o_tmp);
+    const uint32_t nblocks = (uint32_t)((nbk + SORT_BLOCK - 1) / SORT_BLOCK);
+    if (p.qmax) GM_TRY(launch_fill_u32(s, blkcnt, nblocks, 0));
     if (tiled && lds > 65536) {
         // above the default dynamic LDS limit; once per process per kernel
         static const hipError_t attr = [] {
@@ -50,7 +55,7 @@ hipError_t launch_sort(hipStream_t s, const Plan& p, const uint64_t* d_scalars, 
             k_digits<<<blocks_for(p.n_total, 256), 256, 0, s>>>(desc, p.count, p.n_total, d_scalars, p.L, dig, err);
             GM_TRY(hipGetLastError());
             if (p.stile_total > 0) {
-                k_tile_count<<<p.stile_total, 1024, lds, s>>>(desc, p.count, dig, cnt, tbase);
+                k_tile_count<<<p.stile_total, 1024, lds, s>>>(desc, p.count, dig, cnt, tbase, blkcnt, qbase, p.qmax);
                 GM_TRY(hipGetLastError());
             }
         } else {
@@ -65,7 +70,21 @@ hipError_t launch_sort(hipStream_t s, const Plan& p, const uint64_t* d_scalars, 
     GM_TRY(mark(ST_SCAN));
     if (p.n_total > 0) {
         if (tiled) {
-            if (p.stile_total > 0) {
+            if (p.stile_total > 0 && p.qmax) {
+                static const hipError_t attr1 = hipFuncSetAttribute(
+                    reinterpret_cast<const void*>(k_tile_place_blocks), hipFuncAttributeMaxDynamicSharedMemorySize,
+                    SORT_SUB * 10);
+                GM_TRY(attr1);
+                k_tile_place_blocks<<<p.stile_total, 1024, SORT_SUB * 10, s>>>(desc, p.count, dig, off, qbase, p.qmax,
+                                                                              tmp);
+                GM_TRY(hipGetLastError());
+                static const hipError_t attr2 = hipFuncSetAttribute(reinterpret_cast<const void*>(k_block_place),
+                                                                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                                    SORT_STAGE * 8);
+                GM_TRY(attr2);
+                k_block_place<<<nblocks, 1024, SORT_STAGE * 8, s>>>((uint32_t)nbk, off, tmp, sorted);
+                GM_TRY(hipGetLastError());
+            } else if (p.stile_total > 0) {
                 k_tile_place<<<p.stile_total, 1024, lds, s>>>(desc, p.count, dig, off, tbase, sorted);
                 GM_TRY(hipGetLastError());
             }

@@ -124,9 +124,27 @@ __global__ void __launch_bounds__(256) k_scatter(const MsmDesc* __restrict__ des
 //               off[bucket] + tile base, and an LDS atomic per entry returns
 //               its final position.
 // Global atomics drop from one per entry to one per (tile, bucket).
+//
+// Windows of more than SORT_BLOCK buckets place their entries in two passes
+// instead (plan qmax > 0): one LDS counter per bucket leaves every entry's
+// 8-byte store scattered, which cost 0.25 of the 0.30 ms of k_tile_place at
+// G1 2^20. k_tile_count then also gives each tile its slice of every block
+// of SORT_BLOCK consecutive global buckets; k_tile_place_blocks moves the
+// entries to their blocks, k_block_place sorts each block by bucket; both
+// stage their entries in LDS and store whole runs (0.15 ms in all).
 constexpr uint16_t DIG_ZERO = 0xFFFF;
 // digits loaded ahead per thread in the tile kernels
 constexpr int SORT_UNROLL = 8;
+// Buckets per block of the two-pass placement; SORT_QMAX: the most blocks
+// one window of 2^(SORT_LDS_MAX_C - 1) buckets touches (unaligned at both
+// ends).
+constexpr uint32_t SORT_BLOCK = 128;
+// entries of one block sorted in LDS by the second pass (128 KB)
+constexpr uint32_t SORT_STAGE = 16384;
+// points of a sort tile staged per step of the first pass (8 per thread of
+// 1024; 80 KB of LDS with their block numbers)
+constexpr uint32_t SORT_SUB = 8192;
+constexpr uint32_t SORT_QMAX = (1u << (SORT_LDS_MAX_C - 1)) / SORT_BLOCK + 2;
 
 __global__ void __launch_bounds__(256) k_digits(const MsmDesc* __restrict__ desc, int count, uint32_t n_total,
                                                 const uint64_t* __restrict__ scalars, int L,
@@ -163,8 +181,10 @@ __device__ __forceinline__ SortTile sort_tile(const MsmDesc& m, uint32_t gt) {
 
 __global__ void __launch_bounds__(1024) k_tile_count(const MsmDesc* __restrict__ desc, int count,
                                                     const uint16_t* __restrict__ dig, uint32_t* __restrict__ cnt,
-                                                    uint32_t* __restrict__ tbase) {
+                                                    uint32_t* __restrict__ tbase, uint32_t* __restrict__ blkcnt,
+                                                    uint32_t* __restrict__ qbase, uint32_t qmax) {
     extern __shared__ uint32_t lc[];
+    __shared__ uint32_t bs[SORT_QMAX];
     const uint32_t gt = blockIdx.x;
     const MsmDesc& m = desc[GM_FIND(desc, count, stile_base, gt)];
     const SortTile t = sort_tile(m, gt);
@@ -185,12 +205,164 @@ __global__ void __launch_bounds__(1024) k_tile_count(const MsmDesc* __restrict__
             if (code[u] != DIG_ZERO) atomicAdd(&lc[code[u] & 0x7FFFu], 1u);
     }
     __syncthreads();
-    uint32_t* tb = tbase + m.tb_base + (size_t)t.local * m.B;
     uint32_t* gc = cnt + m.bkt_base + (size_t)t.w * m.B;
+    if (qmax == 0) {
+        uint32_t* tb = tbase + m.tb_base + (size_t)t.local * m.B;
+        for (uint32_t b = threadIdx.x; b < m.B; b += blockDim.x) {
+            const uint32_t c = lc[b];
+            tb[b] = c ? atomicAdd(&gc[b], c) : 0u;
+        }
+        return;
+    }
+    // two-pass placement: bucket totals, and the tile's count and base in
+    // every block of SORT_BLOCK global buckets its window touches
+    const uint32_t g0 = m.bkt_base + t.w * m.B;
+    const uint32_t q0 = g0 / SORT_BLOCK;
+    const uint32_t nblk = (g0 + m.B - 1) / SORT_BLOCK - q0 + 1;  // <= B / SORT_BLOCK + 2 <= qmax
+    for (uint32_t q = threadIdx.x; q < nblk; q += blockDim.x) bs[q] = 0;
+    __syncthreads();
     for (uint32_t b = threadIdx.x; b < m.B; b += blockDim.x) {
         const uint32_t c = lc[b];
-        tb[b] = c ? atomicAdd(&gc[b], c) : 0u;
+        if (c) {
+            atomicAdd(&gc[b], c);
+            atomicAdd(&bs[(g0 + b) / SORT_BLOCK - q0], c);
+        }
     }
+    __syncthreads();
+    uint32_t* qb = qbase + (size_t)gt * qmax;
+    for (uint32_t q = threadIdx.x; q < nblk; q += blockDim.x) {
+        const uint32_t c = bs[q];
+        qb[q] = c ? atomicAdd(&blkcnt[q0 + q], c) : 0u;
+    }
+}
+
+// Two-pass placement, pass 1: the tile's entries go to the range of their
+// block of SORT_BLOCK global buckets (the tile's slice of it from
+// k_tile_count). The tile is taken in steps of SORT_SUB points: count the
+// step's entries per block, scan, stage them in LDS in block order, then
+// store each block's run with consecutive lanes on consecutive slots, so the
+// stores are whole lines rather than one scattered 8-byte write per entry.
+__global__ void __launch_bounds__(1024) k_tile_place_blocks(const MsmDesc* __restrict__ desc, int count,
+                                                           const uint16_t* __restrict__ dig,
+                                                           const uint32_t* __restrict__ off,
+                                                           const uint32_t* __restrict__ qbase, uint32_t qmax,
+                                                           uint64_t* __restrict__ tmp) {
+    extern __shared__ uint64_t stage[];  // SORT_SUB entries, then SORT_SUB block numbers (u16)
+    uint16_t* sq = reinterpret_cast<uint16_t*>(stage + SORT_SUB);
+    __shared__ uint32_t gpos[SORT_QMAX], lcnt[SORT_QMAX], lstart[SORT_QMAX], lrun[SORT_QMAX];
+    __shared__ uint32_t step_total;
+    constexpr int SCAN_PER = (SORT_QMAX + 63) / 64;
+    const uint32_t gt = blockIdx.x;
+    const uint32_t tid = threadIdx.x;
+    const MsmDesc& m = desc[GM_FIND(desc, count, stile_base, gt)];
+    const SortTile t = sort_tile(m, gt);
+    const uint32_t g0 = m.bkt_base + t.w * m.B;
+    const uint32_t q0 = g0 / SORT_BLOCK;
+    const uint32_t nblk = (g0 + m.B - 1) / SORT_BLOCK - q0 + 1;
+    const uint32_t* qb = qbase + (size_t)gt * qmax;
+    for (uint32_t q = tid; q < nblk; q += blockDim.x) gpos[q] = off[(q0 + q) * SORT_BLOCK] + qb[q];
+    const uint16_t* dw = dig + (size_t)m.rank_base + (size_t)t.w * m.n;
+    constexpr int PER = SORT_SUB / 1024;  // entries per thread per step
+    for (uint32_t base = t.i0; base < t.i1; base += SORT_SUB) {
+        const uint32_t lim = min(base + SORT_SUB, t.i1);
+        for (uint32_t q = tid; q < nblk; q += blockDim.x) lcnt[q] = 0;
+        __syncthreads();
+        uint32_t code[PER];
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const uint32_t j = base + tid + u * 1024;
+            code[u] = j < lim ? dw[j] : DIG_ZERO;
+        }
+#pragma unroll
+        for (int u = 0; u < PER; ++u)
+            if (code[u] != DIG_ZERO) atomicAdd(&lcnt[(g0 + (code[u] & 0x7FFFu)) / SORT_BLOCK - q0], 1u);
+        __syncthreads();
+        // exclusive scan of the step's block counts by the first wave,
+        // SCAN_PER counts per lane (nblk <= SORT_QMAX <= 64 * SCAN_PER)
+        if (tid < 64) {
+            uint32_t v[SCAN_PER], sum = 0;
+#pragma unroll
+            for (int k = 0; k < SCAN_PER; ++k) {
+                const uint32_t q = tid * SCAN_PER + k;
+                v[k] = q < nblk ? lcnt[q] : 0u;
+                sum += v[k];
+            }
+            uint32_t inc = sum;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t o = __shfl_up(inc, d);
+                if ((int)tid >= d) inc += o;
+            }
+            uint32_t ex = inc - sum;
+#pragma unroll
+            for (int k = 0; k < SCAN_PER; ++k) {
+                const uint32_t q = tid * SCAN_PER + k;
+                if (q < nblk) {
+                    lstart[q] = ex;
+                    lrun[q] = ex;
+                }
+                ex += v[k];
+            }
+            if (tid == 63) step_total = inc;
+        }
+        __syncthreads();
+        const uint32_t total = step_total;
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            if (code[u] == DIG_ZERO) continue;
+            const uint32_t g = g0 + (code[u] & 0x7FFFu);
+            const uint32_t q = g / SORT_BLOCK - q0;
+            const uint32_t at = atomicAdd(&lrun[q], 1u);
+            stage[at] = ((uint64_t)g << 32) | (m.base_off + base + tid + u * 1024) | ((code[u] & 0x8000u) << 16);
+            sq[at] = (uint16_t)q;
+        }
+        __syncthreads();
+        for (uint32_t k = tid; k < total; k += blockDim.x) {
+            const uint32_t q = sq[k];
+            tmp[gpos[q] + (k - lstart[q])] = stage[k];
+        }
+        __syncthreads();
+        for (uint32_t q = tid; q < nblk; q += blockDim.x) gpos[q] += lcnt[q];
+    }
+}
+
+// Pass 2: one workgroup per block of SORT_BLOCK global buckets sorts the
+// block's entries (contiguous after pass 1) by bucket in LDS, one LDS counter
+// per bucket, and writes the block back in its final order: consecutive
+// lanes store consecutive entries, so the stores are whole lines. A block of
+// more than SORT_STAGE entries (few, very full buckets) stores each entry at
+// its final place directly.
+__global__ void __launch_bounds__(1024) k_block_place(uint32_t bkt_total, const uint32_t* __restrict__ off,
+                                                      const uint64_t* __restrict__ tmp,
+                                                      uint64_t* __restrict__ sorted) {
+    extern __shared__ uint64_t stage[];
+    __shared__ uint32_t pos[SORT_BLOCK];
+    const uint32_t gb = blockIdx.x * SORT_BLOCK;
+    const uint32_t ge = min(gb + SORT_BLOCK, bkt_total);
+    const uint32_t e0 = off[gb], e1 = off[ge];
+    const bool staged = e1 - e0 <= SORT_STAGE;
+    for (uint32_t b = threadIdx.x; b < ge - gb; b += blockDim.x) pos[b] = off[gb + b] - (staged ? e0 : 0u);
+    __syncthreads();
+    for (uint32_t j = e0 + threadIdx.x; j < e1; j += SORT_UNROLL * blockDim.x) {
+        uint64_t e[SORT_UNROLL];
+#pragma unroll
+        for (int u = 0; u < SORT_UNROLL; ++u) {
+            const uint32_t k = j + u * blockDim.x;
+            e[u] = k < e1 ? tmp[k] : 0ull;
+        }
+#pragma unroll
+        for (int u = 0; u < SORT_UNROLL; ++u) {
+            if (j + u * blockDim.x >= e1) continue;
+            const uint32_t at = atomicAdd(&pos[(uint32_t)(e[u] >> 32) - gb], 1u);
+            if (staged)
+                stage[at] = e[u];
+            else
+                sorted[at] = e[u];
+        }
+    }
+    if (!staged) return;
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < e1 - e0; j += blockDim.x) sorted[e0 + j] = stage[j];
 }
 
 __global__ void __launch_bounds__(1024) k_tile_place(const MsmDesc* __restrict__ desc, int count,

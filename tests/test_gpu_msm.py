@@ -467,6 +467,15 @@ def test_heavy_buckets_repeated_scalars_and_tiny_top_window():
     for c in (0, 14, 16):
         r = gm.msm_resident(b, s, scalar_bits=299, window=c)
         assert as_point("g1", r[0]) == want, c
+    # one bucket of 2^15 entries per window: its block of 128 buckets holds
+    # more entries than the placement's LDS stage and is placed directly
+    n2 = 1 << 15
+    ks2 = o.gen_u64(96, n2)
+    b2 = gm.Bases.from_k(gm.CURVE_G1, ks2)
+    sr2 = gm.Scalars.upload(np.tile(o.int_to_limbs(s_val, 5), (n2, 1)))
+    t2 = s_val * sum(int(k) for k in ks2) % m.R_ORDER
+    r2 = gm.msm_resident(b2, sr2, scalar_bits=299, window=12)
+    assert as_point("g1", r2[0]) == m.ec_mul(m.FP_OPS, m.G1_STD, t2)
     # tiny top window with random scalars, against the oracle on a subset
     ss = o.gen_scalars_modr(99, n)
     s2 = gm.Scalars.upload(ss)
