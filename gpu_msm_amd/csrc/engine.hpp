@@ -81,6 +81,9 @@ constexpr uint32_t DESC_WSUM_DIRECT = 1u;
 constexpr int SORT_LDS_MAX_C = 16;
 // Points of one window counted by one sort workgroup.
 constexpr uint32_t SORT_TILE_PTS = 1u << 17;
+// Buckets per block of the two-pass placement (windows of more buckets than
+// this place by blocks, then by bucket).
+constexpr uint32_t SORT_BLOCK = 128;
 
 // What a caller asks for: c <= 0 and seg <= 0 pick the defaults.
 struct MsmSpec {

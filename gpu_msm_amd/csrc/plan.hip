@@ -180,11 +180,11 @@ bool make_plan(Plan& p, MsmDesc* d, int curve, int L, int bits, const MsmSpec* s
     if (p.tail_split && 2 * group * seg > 64 * 1024) p.tail_split = false;
     p.stile_total = (uint32_t)stile;
     p.lds_B = lds_sort ? maxB : 0;
-    // Windows of more than 128 buckets place their entries in two passes
-    // (by blocks of 128 buckets, then by bucket), which keeps the runs being
-    // written few enough for L2: one pass with a counter per bucket measured
-    // 0.30 ms at G1 2^20, of which 0.25 ms were its scattered stores.
-    p.qmax = (lds_sort && maxB > 128) ? maxB / 128 + 2 : 0;
+    // Windows of more than SORT_BLOCK buckets place their entries in two
+    // passes (by blocks of SORT_BLOCK buckets, then by bucket), each staged in
+    // LDS so that whole runs are stored: one pass with a counter per bucket
+    // measured 0.30 ms at G1 2^20, of which 0.25 ms were its scattered stores.
+    p.qmax = (lds_sort && maxB > SORT_BLOCK) ? maxB / SORT_BLOCK + 2 : 0;
     p.n_total = (uint32_t)pt;
     p.ent_total = (uint32_t)rank;
     p.bkt_total = (uint32_t)bkt;
@@ -235,7 +235,7 @@ bool make_plan(Plan& p, MsmDesc* d, int curve, int L, int bits, const MsmSpec* s
     p.o_rank = take(p.lds_B ? 0 : rank * 4 + 4);
     p.o_dig = take(p.lds_B ? rank * 2 + 4 : 0);
     p.o_tbase = take(p.lds_B && !p.qmax ? tb * 4 + 4 : 0);
-    p.o_blk = take(p.qmax ? ((bkt + 127) / 128 + 1) * 4 : 0);
+    p.o_blk = take(p.qmax ? ((bkt + SORT_BLOCK - 1) / SORT_BLOCK + 1) * 4 : 0);
     p.o_qbase = take(p.qmax ? stile * p.qmax * 4 + 4 : 0);
     p.o_tmp = take(p.qmax ? rank * 8 + 8 : 0);
     p.o_sorted = take(rank * 8 + 8);

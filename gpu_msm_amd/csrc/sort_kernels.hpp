@@ -135,10 +135,8 @@ __global__ void __launch_bounds__(256) k_scatter(const MsmDesc* __restrict__ des
 constexpr uint16_t DIG_ZERO = 0xFFFF;
 // digits loaded ahead per thread in the tile kernels
 constexpr int SORT_UNROLL = 8;
-// Buckets per block of the two-pass placement; SORT_QMAX: the most blocks
-// one window of 2^(SORT_LDS_MAX_C - 1) buckets touches (unaligned at both
-// ends).
-constexpr uint32_t SORT_BLOCK = 128;
+// SORT_QMAX: the most blocks of SORT_BLOCK buckets one window of
+// 2^(SORT_LDS_MAX_C - 1) buckets touches (unaligned at both ends).
 // entries of one block sorted in LDS by the second pass (128 KB)
 constexpr uint32_t SORT_STAGE = 16384;
 // points of a sort tile staged per step of the first pass (8 per thread of
