@@ -79,9 +79,13 @@ inline void sub(Fp& r, const Fp& a, const Fp& b) {
     }
     memcpy(r.l, t, sizeof t);
 }
+// -a as p - a, and 0 for 0 (reference src/fp.cu:436-443).
 inline void neg(Fp& r, const Fp& a) {
-    Fp z = {{0}};
-    sub(r, z, a);
+    if (is_zero(a)) {
+        r = a;
+        return;
+    }
+    sub(r, P, a);
 }
 // CIOS Montgomery product, canonical output, in x86-64 assembly (BMI2 mulx,
 // ADX adcx/adox: two independent carry chains per row). p's top word is below
@@ -91,7 +95,11 @@ inline void neg(Fp& r, const Fp& a) {
 // one conditional subtraction ends it. Registers r8..r15 hold t and A; after
 // each row t0 is zero, so the names rotate by one instead of shifting words:
 // in row i, t_j is r[8 + (i + j) mod 8] and A is r[8 + (i + 7) mod 8].
-inline void mul(Fp& r, const Fp& a, const Fp& b) {
+// Compiled only for x86-64 and run only where the CPU reports BMI2 and ADX
+// (mul() below); elsewhere the portable product serves.
+#if defined(__x86_64__)
+#define GMH_HAVE_ADX_PRODUCT 1
+inline void mul_adx(Fp& r, const Fp& a, const Fp& b) {
     register uint64_t x8 asm("r8"), x9 asm("r9"), x10 asm("r10"), x11 asm("r11"), x12 asm("r12"),
         x13 asm("r13"), x14 asm("r14"), x15 asm("r15");
     asm volatile(
@@ -467,6 +475,51 @@ inline void mul(Fp& r, const Fp& a, const Fp& b) {
     if (geq_p(t)) sub_p(t);
     memcpy(r.l, t, sizeof r.l);
 }
+#endif
+
+// The same CIOS product in portable C++ (128-bit intermediates), the same
+// no-carry bound: rows stay below 2p in eight words.
+inline void mul_portable(Fp& r, const Fp& a, const Fp& b) {
+    uint64_t t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int i = 0; i < 7; ++i) {
+        u128 c = 0;
+        for (int j = 0; j < 7; ++j) {
+            c += (u128)a.l[j] * b.l[i] + t[j];
+            t[j] = (uint64_t)c;
+            c >>= 64;
+        }
+        t[7] = (uint64_t)c;
+        const uint64_t m = t[0] * PINV;
+        c = ((u128)m * P.l[0] + t[0]) >> 64;
+        for (int j = 1; j < 7; ++j) {
+            c += (u128)m * P.l[j] + t[j];
+            t[j - 1] = (uint64_t)c;
+            c >>= 64;
+        }
+        t[6] = t[7] + (uint64_t)c;
+    }
+    if (geq_p(t)) sub_p(t);
+    memcpy(r.l, t, sizeof r.l);
+}
+
+inline bool cpu_has_adx() {
+#if defined(GMH_HAVE_ADX_PRODUCT)
+    static const bool v = __builtin_cpu_supports("bmi2") && __builtin_cpu_supports("adx");
+    return v;
+#else
+    return false;
+#endif
+}
+
+inline void mul(Fp& r, const Fp& a, const Fp& b) {
+#if defined(GMH_HAVE_ADX_PRODUCT)
+    if (cpu_has_adx()) {
+        mul_adx(r, a, b);
+        return;
+    }
+#endif
+    mul_portable(r, a, b);
+}
 inline void sqr(Fp& r, const Fp& a) { mul(r, a, a); }
 inline void dbl(Fp& r, const Fp& a) { add(r, a, a); }
 inline void to_mont(Fp& r, const Fp& a) { mul(r, a, R2); }
@@ -532,6 +585,7 @@ inline void sqr(Fp2& r, const Fp2& a) {
     add(r.c1, t, t);
 }
 inline bool is_zero(const Fp2& a) { return is_zero(a.c0) && is_zero(a.c1); }
+inline bool eq(const Fp2& a, const Fp2& b) { return eq(a.c0, b.c0) && eq(a.c1, b.c1); }
 inline void inv(Fp2& r, const Fp2& a) {
     Fp t0, t1, n;
     mul(t0, a.c0, a.c0);

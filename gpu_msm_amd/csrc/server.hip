@@ -83,8 +83,9 @@ struct Slot {
 struct HostMlp {
     std::vector<float> P;
     bool ok() const { return P.size() == GM_MLP_PARAMS; }
-    // (c, seg) for one MSM, or (0, 0) to use the heuristic
-    void plan(int curve, int n, int batch, int& c, int& seg) const {
+    // (c, seg) for one MSM of scalars below 2^bits, or (0, 0) to use the
+    // heuristic
+    void plan(int curve, int n, int batch, int bits, int& c, int& seg) const {
         c = 0;
         seg = 0;
         if (!ok()) return;
@@ -111,12 +112,12 @@ struct HostMlp {
         if (!std::isfinite(cc) || !std::isfinite(ls) || cc < 2.0 || cc > 21.0) return;  // heuristic fallback
         // c inside the sweeps' candidate band around the heuristic, at the
         // nearest width whose top window is not sparse
-        const int h = gm::choose_window(curve, std::max(n, 1), 299);
+        const int h = gm::choose_window(curve, std::max(n, 1), bits);
         const int lo = std::max(3, h - 3), hi = std::min(20, h + 2);
         const double ct = std::min((double)hi, std::max((double)lo, cc));
         c = h;
         for (int k = lo; k <= hi; ++k)
-            if (gm::window_ok(299, k) && std::fabs(k - ct) < std::fabs(c - ct)) c = k;
+            if (gm::window_ok(bits, k) && std::fabs(k - ct) < std::fabs(c - ct)) c = k;
         seg = 1 << (int)lround(std::min(5.0, std::max(1.0, ls)));
         seg = std::min(seg, 1 << (c - 1));
     }
@@ -161,7 +162,7 @@ bool plan_batch(gm_server* S, const std::vector<Request*>& rs, gm::Plan& p, std:
     int so = 0;
     for (size_t k = 0; k < rs.size(); ++k) {
         int c = 0, seg = 0;
-        if (S->cfg.planner == GM_PLANNER_MLP) S->mlp.plan(rs[k]->curve, rs[k]->n, (int)rs.size(), c, seg);
+        if (S->cfg.planner == GM_PLANNER_MLP) S->mlp.plan(rs[k]->curve, rs[k]->n, (int)rs.size(), rs[k]->bits, c, seg);
         sp[k] = {rs[k]->n, rs[k]->base_off, so, c, seg};
         so += rs[k]->n;
     }
@@ -308,7 +309,15 @@ void scheduler(gm_server* S) {
             else
                 e = sl.curve == GM_CURVE_G1 ? launch_slot<FpOps>(S, sl) : launch_slot<Fp2Ops>(S, sl);
         }
-        if (e != hipSuccess) sl.fail = true;
+        if (e != hipSuccess) {
+            // Copies and kernels of this batch may already be queued on the
+            // slot's stream: the slot (and its HBM share) is released only
+            // after they have drained, and the completer must not wait on the
+            // event of the slot's previous batch.
+            sl.fail = true;
+            (void)hipGetLastError();
+            if (hipEventRecord(sl.ev, sl.stream) != hipSuccess) (void)hipStreamSynchronize(sl.stream);
+        }
         lk.lock();
         S->launched.push_back(sid);
         S->batches += 1;

@@ -150,6 +150,24 @@ size_t scratch_bytes(int n, int limbs) {
     return gm::al_size((size_t)n * sizeof(Aff<O>)) + p.total;
 }
 
+// Bytes from p to the end of the device allocation that holds it; 0 when the
+// runtime cannot say. A caller that sizes its scratch the reference's way,
+// (blocks + 1) * 16 projective points (reference tests/test_msm.cu:146-155),
+// hands over far less than this pipeline's workspace (24 KB against 1.15 MB
+// for G1 at n = 1000), so the MSM checks the span first and takes
+// stream-ordered memory of its own when the buffer is too short or unknown,
+// instead of writing past it.
+size_t scratch_span(const void* p) {
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    if (hipMemGetAddressRange(&base, &size, const_cast<void*>(p)) != hipSuccess || !base) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    const uintptr_t b = reinterpret_cast<uintptr_t>(base), q = reinterpret_cast<uintptr_t>(p);
+    return (q >= b && q - b < size) ? size - (q - b) : 0;
+}
+
 // One MSM over device-resident ABI records. proj_out selects the ABI
 // projective result (u64 entry points) or the affine record (multi-limb).
 template <class O>
@@ -176,6 +194,7 @@ void msm_device(hipStream_t s, uint32_t gpu, void* d_result, const void* d_point
     if (!plan_one<O>(p, d, n, limbs)) panic(fn, "no pipeline plan for this size");
     const size_t b_aff = gm::al_size((size_t)n * sizeof(Aff<O>));
     uint8_t* scratch = reinterpret_cast<uint8_t*>(d_scratch);
+    if (scratch && scratch_span(scratch) < b_aff + p.total) scratch = nullptr;  // too small: see scratch_span
     const bool own = scratch == nullptr;
     if (own) check(hipMallocAsync(reinterpret_cast<void**>(&scratch), b_aff + p.total, s), fn);
     Aff<O>* d_aff = reinterpret_cast<Aff<O>*>(scratch);

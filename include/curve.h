@@ -1,10 +1,12 @@
 /*
- * C++ device-pointer API of the curve layer: a drop-in for the reference's
- * include/curve.h:140-241 (same names, argument meaning and PointType
- * instantiations; cudaStream_t becomes hipStream_t). Implemented in
- * gpu_msm_amd/csrc/surface.hip on the MI355X kernels of this library.
+ * C++ API of the curve layer: a drop-in for the reference's include/curve.h
+ * (same names, argument meaning and PointType instantiations; cudaStream_t
+ * becomes hipStream_t). The host functions (reference curve.h:50-134) are in
+ * gpu_msm_amd/csrc/host_api.hip; the device-pointer functions (reference
+ * curve.h:140-241) in gpu_msm_amd/csrc/surface.hip on the MI355X kernels of
+ * this library.
  *
- * Every pointer is a DEVICE pointer. `_async` functions enqueue on `stream`
+ * In the device-pointer functions every pointer is a DEVICE pointer. `_async` functions enqueue on `stream`
  * and return; the others also synchronise `stream`. PointType is G1Point or
  * G2Point: affine records in Montgomery form with field values below p.
  * Results are canonical, so they equal the reference's bit for bit.
@@ -23,7 +25,64 @@
 #include <hip/hip_runtime_api.h>
 #include <stddef.h>
 
+#include "fp.h"
+#include "fp2.h"
 #include "gpu_msm_types.h"
+
+/* ------------------------------------------------- host curve functions
+ * reference include/curve.h:50-134 (src/curve.cu:14-778), run on the host by
+ * this library (gpu_msm_amd/csrc/host_api.hip). Coordinates are Montgomery
+ * images except where a function says otherwise; outputs may alias inputs.
+ *
+ * curve_b_g1 / curve_b_g2 return the curves' real coefficients in normal
+ * form, b = 1 and b' = 1 + i, and is_on_curve_g1 / _g2 (normal-form points,
+ * as the reference's fp_mul-based check) test against them: the reference's
+ * constants 4 and 4i (src/curve.cu:280-289) reject its own generators. */
+const Fp& curve_b_g1();
+const Fp2& curve_b_g2();
+bool is_on_curve_g1(const G1Point& point);
+bool is_on_curve_g2(const G2Point& point);
+void g1_point_at_infinity(G1Point& point);  /* (0, 0, infinity) */
+void g2_point_at_infinity(G2Point& point);
+bool g1_is_infinity(const G1Point& point);
+bool g2_is_infinity(const G2Point& point);
+
+/* Affine group law (reference src/curve.cu:72-268): O + P copies P as stored,
+ * P + (-P) and points with y = 0 doubled give O. point_scalar_mul reads
+ * scalar_limbs little-endian u64 words. Instantiated for G1Point and G2Point. */
+template <typename PointType>
+void point_add(PointType& result, const PointType& p1, const PointType& p2);
+template <typename PointType>
+void point_double(PointType& result, const PointType& p);
+template <typename PointType>
+void point_neg(PointType& result, const PointType& p);
+template <typename PointType>
+void point_scalar_mul(PointType& result, const PointType& point, const uint64_t* scalar, int scalar_limbs);
+
+/* Generators in STANDARD form (reference src/curve.cu:840-875 on the host);
+ * callers convert with fp_to_montgomery. */
+const G1Point& g1_generator();
+const G2Point& g2_generator();
+
+/* The device-side generator set-up of the reference (src/curve.cu:816-838);
+ * the same C symbol as rust_wrapper.h declares. */
+extern "C" void init_device_generators(const void* stream, uint32_t gpu_index);
+
+void g1_projective_point_at_infinity(G1ProjectivePoint& p);  /* all zero */
+void g2_projective_point_at_infinity(G2ProjectivePoint& p);
+/* (x, y) -> (x, y, R mod p); O -> all zero */
+void affine_to_projective(G1ProjectivePoint& proj, const G1Point& affine);
+void affine_to_projective(G2ProjectivePoint& proj, const G2Point& affine);
+/* (X / Z, Y / Z); Z = 0 -> O */
+void projective_to_affine_g1(G1Point& affine, const G1ProjectivePoint& proj);
+void projective_to_affine_g2(G2Point& affine, const G2ProjectivePoint& proj);
+/* The reference's standard-projective formulas (src/curve.cu:444-738), so the
+ * raw (X:Y:Z) matches it, not only the affine image; Z = 0 inputs return the
+ * other operand. */
+void projective_point_add(G1ProjectivePoint& result, const G1ProjectivePoint& p1, const G1ProjectivePoint& p2);
+void projective_point_add(G2ProjectivePoint& result, const G2ProjectivePoint& p1, const G2ProjectivePoint& p2);
+void projective_point_double(G1ProjectivePoint& result, const G1ProjectivePoint& p);
+void projective_point_double(G2ProjectivePoint& result, const G2ProjectivePoint& p);
 
 /* ------------------------------------------- single-point operations
  * reference include/curve.h:150-206, src/curve.cu:2386-2525; formulas

@@ -38,13 +38,17 @@ def symbol_table(path: str) -> dict:
 
 def declared_functions() -> list:
     """Function names (with the PointType instantiations for templates) that
-    the C++ headers declare."""
+    the C++ headers declare, once per declaration: an overloaded name (the
+    host and the stream form of fp_add, the G1 and G2 forms of
+    projective_point_add) appears once per overload."""
     names = []
+    ret = r"(?:const\s+\w+\s*&|void|bool|int|size_t|uint64_t)"
     for h in HEADERS:
         txt = open(os.path.join(ROOT, "include", h)).read()
         txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
         txt = re.sub(r"//[^\n]*", "", txt)
-        for mm in re.finditer(r"(template\s*<typename PointType>\s*)?(?:void|bool|int|size_t)\s+(\w+)\s*\(", txt):
+        txt = re.sub(r'extern\s+"C"[^;]*;', "", txt)  # C symbols (rust_wrapper.h)
+        for mm in re.finditer(r"(template\s*<typename PointType>\s*)?" + ret + r"\s+(\w+)\s*\(", txt):
             if mm.group(1):
                 names += [f"{mm.group(2)}<G1Point>", f"{mm.group(2)}<G2Point>"]
             else:
@@ -58,17 +62,28 @@ class Surface:
         self.table = symbol_table(gm.LIB_PATH)
         self._cache = {}
 
-    def mangled(self, name: str) -> str:
+    def matches(self, name: str) -> list:
+        """(demangled, mangled) of every exported overload of `name`."""
         pat = re.compile(r"(^|\s)" + re.escape(name) + r"\(")
-        hits = [m for d, m in self.table.items() if pat.search(d)]
+        return [(d, m) for d, m in self.table.items() if pat.search(d)]
+
+    def mangled(self, name: str, contains: str | None = None) -> str:
+        """The one symbol of `name` whose demangled signature holds
+        `contains`; without it, an overloaded name resolves to its stream
+        form (the overload taking ihipStream_t*)."""
+        hits = self.matches(name)
+        if contains is not None:
+            hits = [h for h in hits if contains in h[0]]
+        elif len(hits) > 1:
+            hits = [h for h in hits if "ihipStream_t" in h[0]]
         if len(hits) != 1:
             raise KeyError(f"{name}: {len(hits)} matching symbols")
-        return hits[0]
+        return hits[0][1]
 
-    def fn(self, name: str, argtypes, restype=None):
-        key = (name, tuple(argtypes), restype)
+    def fn(self, name: str, argtypes, restype=None, contains: str | None = None):
+        key = (name, tuple(argtypes), restype, contains)
         if key not in self._cache:
-            f = getattr(self.lib, self.mangled(name))
+            f = getattr(self.lib, self.mangled(name, contains))
             f.argtypes = list(argtypes)
             f.restype = restype
             self._cache[key] = f
