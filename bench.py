@@ -22,7 +22,6 @@ sys.path.insert(0, ROOT)
 import gpu_msm_amd as gm  # noqa: E402
 from gpu_msm_amd import workloads as wl  # noqa: E402
 
-R_LIMBS = [0x0428001400040001, 0x7BB9B0E8D8CA3461, 0xD04C98CCC4C050BC, 0x7995B34995830FA4, 0x00000511B70539F2]
 # MI355X: 256 CUs x 64 lanes x 2.4 GHz; v_mad_u64_u32 issues at full rate on
 # gfx950 (measured 3.55e13/s with a chain microbenchmark).
 VALU_PEAK_MAD_PER_S = 256 * 64 * 2.4e9
@@ -35,20 +34,12 @@ MAD_PER_MIXED_ADD = 11 * 406
 gen_scalars = wl.uniform_scalars
 
 
-def closed_form_scalar(ks: np.ndarray, ss: np.ndarray) -> int:
-    """t = sum_i s_i k_i mod r; the MSM over P_i = k_i G equals t G."""
-    r = sum(x << (64 * j) for j, x in enumerate(R_LIMBS))
-    acc = 0
-    for j in range(5):
-        col = ss[:, j]
-        # exact sum of 64x64-bit products via 32-bit halves
-        klo, khi = ks & np.uint64(0xFFFFFFFF), ks >> np.uint64(32)
-        clo, chi = col & np.uint64(0xFFFFFFFF), col >> np.uint64(32)
-        s0 = int((klo * clo).astype(object).sum())
-        s1 = int((klo * chi).astype(object).sum()) + int((khi * clo).astype(object).sum())
-        s2 = int((khi * chi).astype(object).sum())
-        acc += (s0 + (s1 << 32) + (s2 << 64)) << (64 * j)
-    return acc % r
+def closed_form_check(curve: int, res: np.ndarray, ks: np.ndarray, ss: np.ndarray) -> bool:
+    """The MSM over P_i = k_i G equals t G with t = sum_i s_i k_i mod r; t
+    and t G are computed on the host (numpy, and the library's host
+    double-and-add), independently of the GPU pipeline."""
+    t = wl.closed_form_scalars(ks, ss, [len(ks)])[0]
+    return res.tobytes() == gm.generator_multiple(curve, t).tobytes()
 
 
 def measured_traffic() -> float | None:
@@ -118,6 +109,11 @@ def bench_batch(reps: int) -> dict:
         graph = bt.run()
     t_graph = (time.perf_counter() - t0) / reps
     ok = plain.tobytes() == eager.tobytes() == graph.tobytes()
+    # every result against its closed form (sum_i s_i k_i mod r) G, on the host
+    idx = np.concatenate([np.arange(b, b + n) for n, b in zip(sizes, boff)])
+    ts = wl.closed_form_scalars(pool_k[idx], sc, sizes)
+    closed = all(graph[j:j + 1].tobytes() == gm.generator_multiple(gm.CURVE_G1, t).tobytes()
+                 for j, t in enumerate(ts))
     n = len(specs)
     # planner A/B on the same batch: analytic heuristic vs autotuned table vs MLP
     ab = planner_ab(pool, s, sizes, boff, soff, graph, reps)
@@ -130,6 +126,7 @@ def bench_batch(reps: int) -> dict:
         "plain_ms": round(t_plain * 1e3, 2), "batched_ms": round(t_eager * 1e3, 2), "graph_ms": round(t_graph * 1e3, 2),
         "workspace_GB": round(bt.workspace_bytes / 1e9, 2),
         "results_equal_alone": ok,
+        "results_closed_form": closed,
         "planners": ab,
     }
     bt.free()
@@ -200,23 +197,48 @@ def bench_served(requests: int) -> dict:
     return out
 
 
+def host_cpu() -> dict:
+    """The host's CPU model (/proc/cpuinfo) and logical CPU count."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count()}
+
+
+def host_share() -> int:
+    """Threads for the host baseline: the CPU share this process is given
+    (OMP_NUM_THREADS, which the GPU hosts set to one GPU's share of their
+    cores), else every logical CPU."""
+    try:
+        n = int(os.environ.get("OMP_NUM_THREADS", "0"))
+    except ValueError:
+        n = 0
+    return max(1, n if n > 0 else (os.cpu_count() or 1))
+
+
 def cpu_baseline(bases_dev, ks: np.ndarray, scalars_np: np.ndarray, curve: str, n_full: int):
-    """Oracle host Pippenger on a bounded sample (N = 2^16, window 16) on all
-    host cores (at most 16), scaled to N = 2^20 by the window-16 cost model
-    (N + 2 * 2^16 projective adds per window); the same sample on one core;
-    and, measured directly, the N = 1024 host configuration on one core and a
-    G2 MSM at N = 2^14 on all cores."""
+    """The oracle's host Pippenger (window 16) on the full workload, measured
+    directly on the process's host CPU share (host_share); a bounded sample
+    on one core (N = 2^16, scaled to the full N by the window-16 cost model,
+    N + 2 * 2^16 projective adds per window, and labelled as scaled); and,
+    measured directly, the N = 1024 host configuration on one core and a G2
+    MSM at N = 2^14 on the share."""
     from oracle import oracle as o
 
     o.build()
-    ns = 1 << 16
-    pts = bases_dev.download(0, ns)
-    threads = min(16, os.cpu_count() or 1)
+    threads = host_share()
+    pts = bases_dev.download(0, n_full)
     t0 = time.perf_counter()
-    o.msm(curve, pts, scalars_np[:ns], c=16, threads=threads)
+    o.msm(curve, pts, scalars_np[:n_full], c=16, threads=threads)
     dt = time.perf_counter() - t0
+    ns = min(1 << 16, n_full)
     t0 = time.perf_counter()
-    o.msm(curve, pts, scalars_np[:ns], c=16, threads=1)
+    o.msm(curve, pts[:ns], scalars_np[:ns], c=16, threads=1)
     dt1 = time.perf_counter() - t0
     t0 = time.perf_counter()
     o.msm("g1", pts[:1024], scalars_np[:1024], threads=1)
@@ -229,15 +251,16 @@ def cpu_baseline(bases_dev, ks: np.ndarray, scalars_np: np.ndarray, curve: str, 
     scale = (n_full + 2 * 65536) / (ns + 2 * 65536)
     lg = int(np.log2(n_full))
     return {
-        "value": round(dt * 1e3 * scale, 1),
+        "value": round(dt * 1e3, 1),
         "unit": "ms",
         "cores": threads,
+        "host_cpu": host_cpu(),
         "kind": "port",
-        "sample": f"oracle Pippenger, {curve.upper()} N=2^16 window 16, {dt:.2f} s measured, scaled x{scale:.2f} "
-                  f"to N=2^{lg}",
+        "sample": f"oracle Pippenger, {curve.upper()} N=2^{lg} window 16, the full workload measured directly "
+                  f"on {threads} threads",
         "single_core": {"value": round(dt1 * 1e3 * scale, 1), "unit": "ms", "cores": 1,
-                        "sample": f"the same N=2^16 sample on one core, {dt1:.2f} s measured, scaled x{scale:.2f} "
-                                  f"to N=2^{lg}"},
+                        "sample": f"N=2^{int(np.log2(ns))} on one core, {dt1:.2f} s measured, scaled x{scale:.2f} "
+                                  f"to N=2^{lg} (window-16 cost model)"},
         "measured_directly": {"g1_n1024_one_core_ms": round(dt_c1 * 1e3, 2),
                               "g2_n2p14_ms": round(dt_g2 * 1e3, 1), "g2_cores": threads},
     }
@@ -278,10 +301,7 @@ def main() -> None:
     # self-check by the closed form: MSM over k_i G equals t G, t computed on the host
     self_check = None
     if not args.no_self_check:
-        t = closed_form_scalar(ks, ss)
-        g = gm.Bases.from_k(gm.CURVE_G1, np.array([1], dtype=np.uint64))
-        tg = gm.msm_resident(g, gm.Scalars.upload(gm.scalars_to_limbs([t])), scalar_bits=299)
-        self_check = res1.tobytes() == tg.tobytes()
+        self_check = closed_form_check(gm.CURVE_G1, res1, ks, ss)
 
     acc_ms = float(np.mean([s["accumulate"] for s in stages]))
     achieved = info["entries"] * MAD_PER_MIXED_ADD / (acc_ms * 1e-3)
@@ -297,9 +317,10 @@ def main() -> None:
     if not args.no_g2:
         print("bench: timing G2 MSM", file=sys.stderr, flush=True)
         b2 = gm.Bases.from_k(gm.CURVE_G2, ks)
-        _, ms2, st2 = time_msm(b2, s1, max(2, args.steps // 2), 1, profile=True)
+        res2, ms2, st2 = time_msm(b2, s1, max(2, args.steps // 2), 1, profile=True)
         g2 = {"ms": round(ms2, 3), "stages_ms": {k: round(float(np.mean([s[k] for s in st2])), 3)
-                                                 for k in gm.STAGES}}
+                                                 for k in gm.STAGES},
+              "self_check_closed_form": None if args.no_self_check else closed_form_check(gm.CURVE_G2, res2, ks, ss)}
         b2.free()
 
     batched = None

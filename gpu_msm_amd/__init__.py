@@ -61,7 +61,7 @@ EXPORTS = [
     "gm_server_destroy",
     "gm_set_profiling", "gm_profile_last", "gm_last_msm_info",
     "gm_point_encoded_size", "gm_points_serialize", "gm_points_deserialize", "gm_points_check",
-    "gm_points_to_canonical", "gm_points_from_canonical",
+    "gm_points_to_canonical", "gm_points_from_canonical", "gm_generator_multiple",
 ]
 
 
@@ -135,6 +135,7 @@ def lib() -> ctypes.CDLL:
         "gm_points_check": ([i, vp, i, i, vp, u32], i),
         "gm_points_to_canonical": ([i, vp, vp, i, u32], i),
         "gm_points_from_canonical": ([i, vp, vp, i, u32], i),
+        "gm_generator_multiple": ([i, vp, i, vp], i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -503,6 +504,17 @@ class Batch:
             self.free()
         except Exception:
             pass
+
+
+def generator_multiple(curve: int, k: int) -> np.ndarray:
+    """k * G for the curve's generator, computed on the host (one affine ABI
+    record, Montgomery form). With bases P_i = k_i G an MSM equals
+    (sum_i s_i k_i mod r) G, so this is an independent host check of a GPU
+    result (workloads.closed_form_scalar gives the exponent)."""
+    kk = int_to_limbs(k % R_ORDER, 5)
+    out = np.zeros(1, dtype=_curve_dtype(curve))
+    _check(lib().gm_generator_multiple(curve, _ptr(kk), 5, _ptr(out)), "gm_generator_multiple")
+    return out
 
 
 def plan_query(curve: int, n: int, limbs: int, bits: int = 0, window: int = 0):

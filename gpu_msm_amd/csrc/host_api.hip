@@ -21,6 +21,7 @@
 #include "../../include/curve.h"
 #include "../../include/fp.h"
 #include "../../include/fp2.h"
+#include "../../include/gpu_msm_ext.h"
 #include "host_field.hpp"
 
 namespace {
@@ -782,4 +783,31 @@ void projective_point_double(G2ProjectivePoint& r, const G2ProjectivePoint& p) {
     ProjF<gmh::Fp2> o;
     proj_double<gmh::Fp2>(o, pj<gmh::Fp2>(p));
     pj_put<gmh::Fp2>(r, o);
+}
+
+// =================================================== extension: closed form
+
+int gm_generator_multiple(int curve, const uint64_t* k, int limbs, void* result_affine) {
+    if (!k || !result_affine || limbs < 1 || limbs > 8) return -1;
+    if (curve == GM_CURVE_G1) {
+        G1Point g = g1_generator(), r;
+        memset(&r, 0, sizeof r);
+        fp_to_montgomery(g.x, g.x);
+        fp_to_montgomery(g.y, g.y);
+        aff_smul<gmh::Fp>(r, g, k, limbs);
+        memcpy(result_affine, &r, sizeof r);
+        return 0;
+    }
+    if (curve == GM_CURVE_G2) {
+        G2Point g = g2_generator(), r;
+        memset(&r, 0, sizeof r);
+        fp_to_montgomery(g.x.c0, g.x.c0);
+        fp_to_montgomery(g.x.c1, g.x.c1);
+        fp_to_montgomery(g.y.c0, g.y.c0);
+        fp_to_montgomery(g.y.c1, g.y.c1);
+        aff_smul<gmh::Fp2>(r, g, k, limbs);
+        memcpy(result_affine, &r, sizeof r);
+        return 0;
+    }
+    return -1;
 }

@@ -8,7 +8,8 @@ else G2; N log-uniform in [2^lo, 2^hi]; bases are slices of resident pools
 
 Reports MSMs/s, server-side p50/p99 latency (submit -> result), client-side
 p50/p99 from the scheduled arrival time, batching statistics, and checks a
-sample of results against the same MSM run alone.
+sample of results against their closed form computed on the host and against
+the same MSM run alone.
 """
 from __future__ import annotations
 
@@ -38,6 +39,7 @@ class Stream:
         rng = np.random.default_rng(seed)
         self.pool_n = 1 << log2_pool
         k = wl.uniform_k(rng, self.pool_n)
+        self.k = k  # pool bases are k_i G: results have the closed form (sum s_i k_i) G
         self.g1 = gm.Bases.from_k(gm.CURVE_G1, k)
         self.g2 = gm.Bases.from_k(gm.CURVE_G2, k)
         self.scalars = wl.uniform_scalars(rng, scalar_pool)
@@ -55,7 +57,10 @@ class Stream:
 
 def run(stream: Stream, rate: float, count: int, lo: int, hi: int, seed: int = 5, slots: int = 3,
         max_batch: int = 64, budget: int = 48 << 30, planner: int = gm.PLANNER_HEURISTIC, mlp_params=None,
-        check: int = 16, log=None) -> dict:
+        check: int = 16, log=None, results_out: list | None = None) -> dict:
+    """Serves `count` requests; with results_out, appends (curve, n,
+    base_offset, scalars, rc, affine record) per request for the caller to
+    check."""
     rng, curves, sizes, boff = make_stream(seed, count, lo, hi, stream.pool_n)
     arr = np.cumsum(rng.exponential(1.0 / rate, size=count))  # scheduled arrivals (s)
     scal = [stream.scalar_slice(rng, int(n)) for n in sizes]
@@ -91,13 +96,19 @@ def run(stream: Stream, rate: float, count: int, lo: int, hi: int, seed: int = 5
     st = srv.stats()
     srv.close()
     ok = sum(1 for rc, _ in results if rc == 0)
-    # sampled check: the same MSM run alone gives the same affine point
+    if results_out is not None:
+        for i in range(count):
+            results_out.append((int(curves[i]), int(sizes[i]), int(boff[i]), scal[i], results[i][0], results[i][1]))
+    # sampled checks: the closed form (sum_i s_i k_i mod r) G computed on the
+    # host, and the same MSM run alone on the GPU
     idx = np.linspace(0, count - 1, min(check, count)).astype(int)
-    same = True
+    same = closed = True
     for i in idx:
+        n, b, cv = int(sizes[i]), int(boff[i]), int(curves[i])
+        t = wl.closed_form_scalars(stream.k[b:b + n], scal[i], [n])[0]
+        closed &= results[i][0] == 0 and results[i][1].tobytes() == gm.generator_multiple(cv, t).tobytes()
         sc = gm.Scalars.upload(scal[i])
-        alone = gm.msm_resident(stream.pool(int(curves[i])), sc, n=int(sizes[i]), base_offset=int(boff[i]),
-                                scalar_bits=299)
+        alone = gm.msm_resident(stream.pool(cv), sc, n=n, base_offset=b, scalar_bits=299)
         sc.free()
         same &= results[i][0] == 0 and alone.tobytes() == results[i][1].tobytes()
     return {
@@ -111,7 +122,7 @@ def run(stream: Stream, rate: float, count: int, lo: int, hi: int, seed: int = 5
         "peak_inflight_GB": round(st["peak_inflight_bytes"] / 1e9, 3),
         "g2_fraction": round(float(np.mean(curves == gm.CURVE_G2)), 3),
         "mean_n": int(np.mean(sizes)),
-        "sampled_same_alone": bool(same), "sampled": int(len(idx)),
+        "sampled_closed_form": bool(closed), "sampled_same_alone": bool(same), "sampled": int(len(idx)),
     }
 
 

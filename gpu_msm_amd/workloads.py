@@ -6,6 +6,8 @@ server and the tests). Inputs only; no reference data.
 - ``batch_workload``: the mixed-size batch — MSM sizes log-uniform in
   [2^lo, 2^hi], bases as slices of one resident pool at seeded offsets, one
   fresh scalar per point.
+- ``closed_form_scalars``: for bases P_i = k_i G, the exponent
+  t = sum_i s_i k_i mod r of each MSM, so that its result is t G.
 """
 from __future__ import annotations
 
@@ -59,3 +61,38 @@ def specs_from(sizes, base_off, scal_off, windows=None, segs=None):
         s = 0 if segs is None else int(segs[k])
         out.append((int(sizes[k]), int(base_off[k]), int(scal_off[k]), w, s))
     return out
+
+
+R_ORDER = sum(v << (64 * j) for j, v in enumerate(R_LIMBS))
+
+
+def closed_form_scalars(k: np.ndarray, s: np.ndarray, sizes) -> list:
+    """t_j = sum_i s_i k_i mod r for consecutive MSMs: MSM j owns the next
+    sizes[j] rows of k (u64) and s (rows of u64 limbs). Exact in numpy: every
+    64 x 64-bit limb product is formed from 32-bit halves, each partial
+    product split into 32-bit halves whose column sums stay below 2^64 for
+    fewer than 2^32 rows."""
+    k = np.ascontiguousarray(k, dtype=np.uint64)
+    s = np.ascontiguousarray(s, dtype=np.uint64).reshape(len(k), -1)
+    sizes = np.asarray(sizes, dtype=np.int64)
+    assert sizes.sum() == len(k)
+    starts = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    nz = sizes > 0
+    m32, sh32 = np.uint64(0xFFFFFFFF), np.uint64(32)
+    kl, kh = k & m32, k >> sh32
+    acc = [0] * len(sizes)
+
+    def seg_sum(v: np.ndarray) -> np.ndarray:
+        out = np.zeros(len(sizes), dtype=np.uint64)
+        if len(v):
+            out[nz] = np.add.reduceat(v, starts[nz])
+        return out
+
+    for j in range(s.shape[1]):
+        sl, sh = s[:, j] & m32, s[:, j] >> sh32
+        for a, b, w in ((kl, sl, 0), (kl, sh, 32), (kh, sl, 32), (kh, sh, 64)):
+            p = a * b
+            lo, hi = seg_sum(p & m32), seg_sum(p >> sh32)
+            for q in range(len(sizes)):
+                acc[q] += (int(lo[q]) + (int(hi[q]) << 32)) << (64 * j + w)
+    return [v % R_ORDER for v in acc]

@@ -159,7 +159,11 @@ void point_to_montgomery_batch(hipStream_t stream, uint32_t gpu_index, PointType
  * library's pipeline (sorted bucket lists, per-window buckets) needs the byte
  * count returned by the *_scratch_bytes query for the same n (and limbs).
  * A caller that cannot size it passes d_scratch = NULL: the call then takes
- * and returns the memory stream-ordered (hipMallocAsync / hipFreeAsync).
+ * and returns the memory stream-ordered (hipMallocAsync / hipFreeAsync). A
+ * non-NULL d_scratch is used only when the bytes from it to the end of its
+ * device allocation (hipMemGetAddressRange) cover that count; a shorter
+ * buffer, such as one sized the reference's way, is left untouched and the
+ * call takes stream-ordered memory as for NULL.
  *
  * The _async forms never block the host: the window sums go to pinned host
  * memory, a host function enqueued on the stream (hipLaunchHostFunc) runs the
@@ -170,6 +174,13 @@ void point_to_montgomery_batch(hipStream_t stream, uint32_t gpu_index, PointType
  * Calls on different streams may overlap when their scratch buffers differ.
  * The _async forms are not meant for hipGraph capture (their pinned slot is
  * taken when the call is made). */
+/* The reference's legacy sizing constants (reference include/curve.h:143-145),
+ * kept so callers that size scratch with them still compile; such a buffer is
+ * too small for this pipeline and the MSM then takes its own (above). */
+#define MSM_WINDOW_SIZE 4
+#define MSM_BUCKET_COUNT 16
+#define MSM_SIGNED_BUCKET_COUNT 8
+
 size_t point_msm_u64_scratch_bytes_g1(int n);
 size_t point_msm_u64_scratch_bytes_g2(int n);
 template <typename PointType>

@@ -164,6 +164,16 @@ int gm_server_inject_failures(gm_server* s, int batches);
 /* Drains queued and in-flight requests, then stops. */
 void gm_server_destroy(gm_server* s);
 
+/* ------------------------------------------------------- host closed form */
+
+/* k * G on the host for the curve's generator G (G1 or G2), k given as
+ * `limbs` little-endian u64 words: the affine ABI record (G1Point / G2Point,
+ * Montgomery form, canonical; O as (0, 0, infinity)). A workload whose bases
+ * are P_i = k_i G has MSM result (sum_i s_i k_i mod r) G, so this checks a
+ * GPU MSM against an independent host computation. Returns 0, or -1 for a
+ * bad curve / limbs outside [1, 8]. */
+int gm_generator_multiple(int curve, const uint64_t* k, int limbs, void* result_affine);
+
 /* -------------------------------------------------------------- profiling */
 
 /* Record HIP events between pipeline stages of subsequent MSMs on this GPU. */

@@ -1,6 +1,6 @@
 """Compiles the caller harnesses against the built library:
-rust_flow (C11 with gcc, on include/rust_wrapper.h) and device_api (C++17 with
-g++ on include/curve.h and the HIP runtime API headers). Binaries go to
+rust_flow (C11 with gcc, on include/rust_wrapper.h), device_api and msm_flow
+(C++17 with g++ on include/curve.h and the HIP runtime API headers). Binaries go to
 tests/harness/_build/ and find libgpu_msm_amd.so through an $ORIGIN rpath, so
 the tree can move.
 
@@ -24,8 +24,11 @@ COMMANDS = {
     "device_api": ["g++", "-std=c++17", "-O1", "-Wall", "-Wextra", "-Werror", "-D__HIP_PLATFORM_AMD__",
                    "-I", INC, "-I", f"{ROCM}/include", os.path.join(HERE, "device_api.cpp"), "-o",
                    os.path.join(OUT, "device_api"), *LINK, "-L", f"{ROCM}/lib", "-lamdhip64"],
+    "msm_flow": ["g++", "-std=c++17", "-O1", "-Wall", "-Wextra", "-Werror", "-D__HIP_PLATFORM_AMD__",
+                 "-I", INC, "-I", f"{ROCM}/include", os.path.join(HERE, "msm_flow.cpp"), "-o",
+                 os.path.join(OUT, "msm_flow"), *LINK, "-L", f"{ROCM}/lib", "-lamdhip64"],
 }
-SOURCES = {"rust_flow": "rust_flow.c", "device_api": "device_api.cpp"}
+SOURCES = {"rust_flow": "rust_flow.c", "device_api": "device_api.cpp", "msm_flow": "msm_flow.cpp"}
 
 
 def build(name: str) -> str:

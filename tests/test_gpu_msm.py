@@ -481,3 +481,31 @@ def test_heavy_buckets_repeated_scalars_and_tiny_top_window():
     s2 = gm.Scalars.upload(ss)
     full = gm.msm_resident(b, s2, scalar_bits=299, window=14)
     assert same(full, gm.msm_resident(b, s2, scalar_bits=299, window=11))
+
+
+@pytest.mark.parametrize("curve", ["g1", "g2"])
+def test_global_sort_path_wide_windows(curve):
+    """Windows wider than 16 bits place entries with one global atomic per
+    entry (the sort's fallback path) instead of LDS tile counters: windows of
+    17 and 20 bits on scalars below r, 24 bits on u64 scalars (3 windows of
+    2^23 buckets), alone and in a batch that mixes c = 12 with c = 18 (the
+    whole batch then takes the global path), against the oracle."""
+    n = 1500 if curve == "g1" else 500
+    ks, b = _pool(curve, n, 140 if curve == "g1" else 141)
+    bases = o.bases_from_k(curve, ks)
+    ss = o.gen_scalars_modr(142, n)
+    s = gm.Scalars.upload(ss)
+    want = o.msm(curve, bases, ss)
+    for c in (17, 20):
+        print(f"{curve} n={n} window={c}", flush=True)
+        assert same(gm.msm_resident(b, s, scalar_bits=299, window=c), want), c
+    su = o.gen_u64(143, n)
+    s64 = gm.Scalars.upload(su.reshape(-1, 1))
+    print(f"{curve} n={n} window=24 u64 scalars", flush=True)
+    assert same(gm.msm_resident(b, s64, scalar_bits=64, window=24), o.msm(curve, bases, su.reshape(-1, 1)))
+    specs = [(n // 2, 0, 0, 12), (n - n // 2, n // 2, n // 2, 18), (n // 3, n // 5, 0, 12)]
+    print(f"{curve} batch of windows 12 / 18 / 12", flush=True)
+    got = gm.msm_batch(b, s, specs, scalar_bits=299)
+    for k, (m_, bo, so, _) in enumerate(specs):
+        w = o.msm(curve, bases[bo:bo + m_], ss[so:so + m_])
+        assert same(got[k:k + 1], w), k

@@ -1,8 +1,10 @@
 """GPU parity of the reference's C++ device-pointer surface: include/curve.h
 (point operations, Montgomery batch conversion, device-pointer MSM), fp.h /
 fp2.h (stream element operations) and fp_kernels.h / fp2_kernels.h (arrays and
-host-pointer operations), called by mangled name through ctypes, plus the two
-caller harnesses (the Rust layer's C ABI flow and a C++ device-API caller).
+host-pointer operations), called by mangled name through ctypes, plus the
+caller harnesses (the Rust layer's C ABI flow, a C++ device-API caller, and
+the reference's own tests/test_msm.cu flow). Device buffers and the stream
+come from the HIP runtime directly (tests/hipmem.py).
 
 Expected values come from the big-integer model (tests/pymodel.py: textbook
 affine group law, Python modular arithmetic) and the oracle's MSM; every
@@ -16,6 +18,7 @@ import numpy as np
 import pytest
 
 import gpu_msm_amd as gm
+import hipmem as hm
 import pymodel as m
 from cxx_surface import I32, P, U32, U64, Surface
 from oracle import oracle as o
@@ -24,33 +27,19 @@ sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "har
 import build as harness  # noqa: E402
 
 pytestmark = pytest.mark.gpu
-torch = pytest.importorskip("torch")
 
 MASK64 = (1 << 64) - 1
 
-
 @pytest.fixture(scope="module")
 def S():
-    if not torch.cuda.is_available():
-        pytest.skip("no GPU")
+    hm.require_gpu()
     return Surface()
 
 
-def stream():
-    return torch.cuda.current_stream().cuda_stream
-
-
-def to_dev(arr: np.ndarray):
-    return torch.from_numpy(np.ascontiguousarray(arr).view(np.uint8).reshape(-1).copy()).to("cuda")
-
-
-def to_host(t, dtype) -> np.ndarray:
-    torch.cuda.synchronize()
-    return t.cpu().numpy().view(dtype)
-
-
-def empty_dev(nbytes: int):
-    return torch.zeros(max(nbytes, 8), dtype=torch.uint8, device="cuda")
+stream = hm.stream
+to_dev = hm.to_dev
+to_host = hm.to_host
+empty_dev = hm.empty_dev
 
 
 # ------------------------------------------------------------ ABI records
@@ -561,3 +550,46 @@ def test_device_api_harness_vs_oracle(tmp_path, curve, n):
     got = np.fromfile(out, dtype=CURVES[curve]["dt"])
     want = o.msm(curve, bases, sc.reshape(-1, 1))
     assert fields(got) == fields(want)
+
+
+def test_msm_flow_harness_reference_tests(tmp_path):
+    """The reference's tests/test_msm.cu flow call for call (host
+    g1_generator / g2_generator, fp_to_montgomery in place, scratch sized as
+    (blocks + 1) * MSM_BUCKET_COUNT points, point_msm_u64_*, the expected
+    N(N+1)/2 * G by point_scalar_mul_u64, host projective_to_affine, device
+    point_from_montgomery, fp_cmp): N = 10 (G1, G2), 100 (G1), then 1000 and
+    5000 on both curves, whose reference-sized scratch is far too small for
+    this pipeline; guard buffers around it must stay untouched."""
+    exe = harness.build("msm_flow")
+    print("msm_flow default cases", flush=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "ALL PASS" in r.stdout and r.stdout.count("PASS (") == 7, r.stdout
+
+
+@pytest.mark.parametrize("curve,n", [("g1", 3000), ("g2", 700)])
+def test_point_msm_u64_reference_sized_scratch_vs_oracle(S, curve, n):
+    """A d_scratch sized the reference's way ((blocks + 1) * 16 projective
+    points, reference tests/test_msm.cu:146-155) is shorter than this
+    pipeline's workspace: the MSM must not write past it (it takes
+    stream-ordered memory of its own) and still equal the oracle. A buffer
+    allocated right after the scratch must come back unchanged."""
+    ks = o.gen_u64(3300 + n, n)
+    bases = o.bases_from_k(curve, ks)
+    sc = o.gen_u64(3301 + n, n)
+    want = o.msm(curve, bases, sc.reshape(-1, 1))
+    proj_dt = o.G1P_DT if curve == "g1" else o.G2P_DT
+    ref_bytes = ((n + 255) // 256 + 1) * 16 * proj_dt.itemsize
+    need = S.fn(f"point_msm_u64_scratch_bytes_{curve}", [I32], ctypes.c_size_t)(n)
+    assert ref_bytes < need
+    d_b, d_s = to_dev(bases), to_dev(sc)
+    scratch = empty_dev(ref_bytes, 0x5A)
+    guard = empty_dev(1 << 20, 0x5A)
+    for name in (f"point_msm_u64_{curve}", f"point_msm_u64_async_{curve}"):
+        f = S.fn(name, [P, U32, P, P, P, P, I32])
+        d_out = empty_dev(proj_dt.itemsize)
+        print(f"{name} n={n} reference-sized scratch {ref_bytes} B", flush=True)
+        f(stream(), 0, d_out.data_ptr(), d_b.data_ptr(), d_s.data_ptr(), scratch.data_ptr(), n)
+        got = o.to_affine(curve, to_host(d_out, proj_dt))
+        assert fields(got) == fields(want), name
+    assert (to_host(guard, np.uint8) == 0x5A).all()

@@ -65,7 +65,7 @@ def test_templates_are_the_point_type_instantiations(S):
         assert sym.startswith("_Z9point_msmI") and f"I7{t}E" in sym
 
 
-@pytest.mark.parametrize("name", ["rust_flow", "device_api"])
+@pytest.mark.parametrize("name", ["rust_flow", "device_api", "msm_flow"])
 def test_harness_compiles_and_links(S, name):
     exe = harness.build(name)
     assert os.access(exe, os.X_OK)
