@@ -62,6 +62,7 @@ EXPORTS = [
     "gm_set_profiling", "gm_profile_last", "gm_last_msm_info",
     "gm_point_encoded_size", "gm_points_serialize", "gm_points_deserialize", "gm_points_check",
     "gm_points_to_canonical", "gm_points_from_canonical", "gm_generator_multiple",
+    "gm_point_msm_noalloc",
 ]
 
 

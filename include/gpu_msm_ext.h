@@ -164,6 +164,24 @@ int gm_server_inject_failures(gm_server* s, int batches);
 /* Drains queued and in-flight requests, then stops. */
 void gm_server_destroy(gm_server* s);
 
+/* ------------------------------------------------- MSM without allocation */
+
+/* No counterpart in the reference. For callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): the MSM of point_msm_async<PointType> (curve.h) over device
+ * pointers, d_points n affine ABI records (G1Point / G2Point) in Montgomery
+ * form, d_scalars n x scalar_limbs little-endian u64 words, d_result one
+ * canonical affine ABI record, all enqueued on `stream` (a hipStream_t; NULL
+ * for the null stream) with no host wait. It calls no hipMalloc / hipFree or
+ * allocating library routine: the workspace is one fixed 96 MiB device array
+ * per GPU, held until the process ends, and the window combine runs on the
+ * device. Sizes whose workspace does not fit (about 2^15 G1 or 2^14 G2 points
+ * with 5-limb scalars) return -6. Calls that overlap on one GPU share that
+ * array and race: serialise them. Returns 0; -1 bad arguments; -2 device or
+ * set-up failure; -4 launch failure; -6 too large for the fixed workspace. */
+int gm_point_msm_noalloc(void* stream, uint32_t gpu_index, int curve, void* d_result, const void* d_points,
+                         const uint64_t* d_scalars, int scalar_limbs, int n);
+
 /* ------------------------------------------------------- host closed form */
 
 /* k * G on the host for the curve's generator G (G1 or G2), k given as
