@@ -42,10 +42,10 @@ def closed_form_check(curve: int, res: np.ndarray, ks: np.ndarray, ss: np.ndarra
     return res.tobytes() == gm.generator_multiple(curve, t).tobytes()
 
 
-def measured_traffic() -> float | None:
-    """HBM bytes per k_accumulate launch from the committed rocprofv3
-    FETCH_SIZE/WRITE_SIZE summary (profiles/k_accumulate_g1_2p20.md)."""
-    path = os.path.join(ROOT, "profiles", "k_accumulate_g1_2p20.md")
+def measured_traffic(name: str = "k_accumulate_g1_2p20.md") -> float | None:
+    """HBM bytes per k_accumulate launch from a committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary under profiles/."""
+    path = os.path.join(ROOT, "profiles", name)
     try:
         for line in open(path):
             if line.startswith("traffic_bytes_per_launch:"):
@@ -318,8 +318,24 @@ def main() -> None:
         print("bench: timing G2 MSM", file=sys.stderr, flush=True)
         b2 = gm.Bases.from_k(gm.CURVE_G2, ks)
         res2, ms2, st2 = time_msm(b2, s1, max(2, args.steps // 2), 1, profile=True)
+        info2 = gm.last_msm_info()
+        acc2 = float(np.mean([s["accumulate"] for s in st2]))
+        ach2 = info2["entries"] * 3 * MAD_PER_MIXED_ADD / (acc2 * 1e-3)
         g2 = {"ms": round(ms2, 3), "stages_ms": {k: round(float(np.mean([s[k] for s in st2])), 3)
                                                  for k in gm.STAGES},
+              "roofline": {
+                  "kernel": "k_accumulate (G2 bucket accumulation, mixed XYZZ adds over Fp2 lane pairs)",
+                  "bound": "valu",
+                  "achieved": round(ach2 / 1e12, 3),
+                  "peak": round(VALU_PEAK_MAD_PER_S / 1e12, 3),
+                  "unit": "T v_mad_u64_u32/s (algorithmic: 3 x 11 M x 406 mad per mixed add, Karatsuba Fp2)",
+                  "frac": round(ach2 / VALU_PEAK_MAD_PER_S, 4),
+                  "traffic": measured_traffic("k_accumulate_g2_2p20.md"),
+                  "hbm_algorithmic_GBs": round((info2["entries"] * (256 + 8) + info2["windows"]
+                                                * (1 << (info2["window"] - 1)) * (8 + 512)) / (acc2 * 1e-3) / 1e9, 1),
+                  "launch_ms": round(acc2, 3),
+                  "mixed_adds_per_launch": info2["entries"],
+              },
               "self_check_closed_form": None if args.no_self_check else closed_form_check(gm.CURVE_G2, res2, ks, ss)}
         b2.free()
 

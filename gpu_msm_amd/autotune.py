@@ -119,7 +119,7 @@ def training_set(records, tol: float = 0.02):
         for q in rs:
             if q["ms_per_msm"] <= t0 * (1 + tol):
                 xs.append(pl.features(lg, b, cv - 1))
-                ts.append(pl.targets(q["c"], q["seg"]))
+                ts.append(pl.targets(q["c"], q["seg"], q["heuristic_c"]))
     return np.array(xs, dtype=np.float32), np.array(ts, dtype=np.float32)
 
 

@@ -405,6 +405,30 @@ __device__ __forceinline__ void fp_cond_sub(uint32_t (&t)[NL], const uint32_t (&
     for (int i = 0; i < NL; ++i) t[i] = keep ? t[i] : d[i];
 }
 
+// 8p in 28-bit limbs, the top limb holding 29 bits (8p > R)
+constexpr uint32_t P8L[NL] = {0x5855558, 0xe001355, 0xdae6188, 0x29462b9, 0xf72b772, 0x1f11d66,
+                              0x412736e, 0xd638b01, 0xbad325d, 0x17f1907, 0xd25bc50, 0xb20004a,
+                              0xa4b2d56, 0xa9007e1, 0x9462f29, 0x1e6f707d};
+
+// Brings a reduced value in (-4p, 5p) (28-bit limbs 0..14 and a signed top
+// limb) to [0, 2p): add 4p when negative (now in [0, 5p)), then subtract 4p
+// and 2p where that does not borrow.
+__device__ __forceinline__ void redc_fold_wide(Fp& r, uint32_t (&t)[NL], int64_t top) {
+    const uint32_t msk = top < 0 ? 0xFFFFFFFFu : 0u;
+    int32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < NL - 1; ++i) {
+        const int32_t v = (int32_t)t[i] + (int32_t)(P4L[i] & msk) + c;
+        t[i] = (uint32_t)v & MASK28;
+        c = v >> 28;
+    }
+    t[NL - 1] = (uint32_t)((int32_t)top + (int32_t)(P4L[NL - 1] & msk) + c);
+    fp_cond_sub(t, P4L);
+    fp_cond_sub(t, P2L);
+#pragma unroll
+    for (int i = 0; i < NL; ++i) r.v[i] = t[i];
+}
+
 // r = a^2 R^{-1} - b - 2c for a raw a and b, c in [0, 2p), in [0, 2p) (the
 // X3 of every XYZZ formula). b + 2c - 6p is subtracted from the high half of
 // the square's column sums, i.e. (b + 2c - 6p) R before the reduction: the
@@ -742,10 +766,13 @@ __device__ __forceinline__ void fp2_lane_mul(Fp& r, const Fp& a, const Fp& ao, c
     redc_fold(r, t, top, true);
 }
 
-// Own component of a b - c d with one reduction: both parts lie in
-// (-8p^2, 8p^2) before and in (-2p, 3p) after the reduction.
+// Own component of a b - c d with one reduction. With every operand in
+// [0, 2p) both parts lie in (-8p^2, 8p^2) before and in (-2p, 3p) after the
+// reduction; with a and b raw (magnitude below 2p) in (-16p^2, 16p^2) and
+// (-4p, 5p), which `wide` folds.
 __device__ __forceinline__ void fp2_lane_mul_sub(Fp& r, const Fp& a, const Fp& ao, const Fp& b, const Fp& bo,
-                                                 const Fp& c, const Fp& co, const Fp& d, const Fp& dox) {
+                                                 const Fp& c, const Fp& co, const Fp& d, const Fp& dox,
+                                                 bool wide = false) {
     int32_t y0[NL], w0[NL], y1[NL], w1[NL];
     lane_operands(y0, w0, b, bo, false);
     lane_operands(y1, w1, d, dox, true);
@@ -754,13 +781,37 @@ __device__ __forceinline__ void fp2_lane_mul_sub(Fp& r, const Fp& a, const Fp& a
     uint32_t t[NL];
     int64_t top;
     redc_signed<4>(t, top, x, yy);
-    redc_fold(r, t, top, true);
+    if (wide)
+        redc_fold_wide(r, t, top);
+    else
+        redc_fold(r, t, top, true);
+}
+
+// Own component of a^2 for a raw a (limbs of magnitude below 2^28, value of
+// magnitude below 2p): part 0 (a0 + a1)(a0 - a1) = a0^2 - a1^2 in
+// (-4p^2, 4p^2), part 1 (a1 + a1) a0 in (-8p^2, 8p^2), one signed product per
+// lane (limb products below 2^58, a column below 2^63). The reduced value
+// lies in (-2p, 3p); returned as t / top before the fold.
+__device__ __forceinline__ void fp2_lane_sqr_cols(uint32_t (&t)[NL], int64_t& top, const Fp& a) {
+    Fp ao, x;
+    fp_pair_swap(ao, a);
+    const bool q = lane_part() != 0;
+    int32_t y[NL];
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+        const int32_t av = (int32_t)a.v[i], bv = (int32_t)ao.v[i];
+        x.v[i] = (uint32_t)(av + (q ? av : bv));
+        y[i] = q ? bv : av - bv;
+    }
+    const Fp* const xs[1] = {&x};
+    const int32_t(*const ys[1])[NL] = {&y};
+    redc_signed<1>(t, top, xs, ys);
 }
 
 struct Fp2LaneOps {
     using F = Fp;
     static constexpr int WORDS = NL;
-    static constexpr bool RAW = false;
+    static constexpr bool RAW = true;  // has sqr_sub2 / sqr_sub2z
     __device__ static __forceinline__ void add(F& r, const F& a, const F& b) { fp_add(r, a, b); }
     __device__ static __forceinline__ void sub(F& r, const F& a, const F& b) { fp_sub(r, a, b); }
     __device__ static __forceinline__ void dbl(F& r, const F& a) { fp_dbl(r, a); }
@@ -794,12 +845,55 @@ struct Fp2LaneOps {
         fp_pair_swap(dx, d);
         fp2_lane_mul_sub(r, a, ao, b, bo, c, co, d, dx);
     }
-    // no raw form: the "raw" operations are the normalised ones
-    __device__ static __forceinline__ void sub_raw(F& r, const F& a, const F& b) { sub(r, a, b); }
-    __device__ static __forceinline__ void sqr_raw(F& r, const F& a) { sqr(r, a); }
+    // Raw differences, as FpOps has them: a difference that only feeds
+    // products stays limb by limb (signed limbs, value of magnitude below
+    // 2p), and the lane products take signed operands anyway. mul_raw is the
+    // general lane product (raw operands keep each part below 8p^2);
+    // mul_sub_raw (raw a, b) and the fused X3 fold wider ranges.
+    __device__ static __forceinline__ void sub_raw(F& r, const F& a, const F& b) { fp_sub_raw(r, a, b); }
+    __device__ static __forceinline__ void sqr_raw(F& r, const F& a) {
+        uint32_t t[NL];
+        int64_t top;
+        fp2_lane_sqr_cols(t, top, a);
+        redc_fold(r, t, top, true);
+    }
     __device__ static __forceinline__ void mul_raw(F& r, const F& a, const F& b) { mul(r, a, b); }
     __device__ static __forceinline__ void mul_sub_raw(F& r, const F& a, const F& b, const F& c, const F& d) {
-        mul_sub(r, a, b, c, d);
+        Fp ao, bo, co, dx;
+        fp_pair_swap(ao, a);
+        fp_pair_swap(bo, b);
+        fp_pair_swap(co, c);
+        fp_pair_swap(dx, d);
+        fp2_lane_mul_sub(r, a, ao, b, bo, c, co, d, dx, true);
+    }
+    // Own component of a^2 - b - 2c (a raw, b and c in [0, 2p)): the square
+    // reduces to (-2p, 3p); 8p - b - 2c (limbs 8 p_i - b_i - 2 c_i, in
+    // (-3 * 2^28, 2^31)) is added with the carries, so the sum lies in
+    // (0, 11p), and subtracting 8p, 4p and 2p where that does not borrow
+    // brings it to [0, 2p).
+    __device__ static __forceinline__ void sqr_sub2(F& r, const F& a, const F& b, const F& c) {
+        uint32_t t[NL];
+        int64_t top;
+        fp2_lane_sqr_cols(t, top, a);
+        int64_t acc = 0;
+#pragma unroll
+        for (int i = 0; i < NL - 1; ++i) {
+            acc += (int64_t)t[i] + ((int64_t)(int32_t)(8u * PL[i]) - (int32_t)b.v[i] - 2 * (int64_t)(int32_t)c.v[i]);
+            t[i] = (uint32_t)acc & MASK28;
+            acc >>= 28;
+        }
+        acc += top + ((int64_t)(8u * PL[NL - 1]) - (int32_t)b.v[NL - 1] - 2 * (int64_t)(int32_t)c.v[NL - 1]);
+        t[NL - 1] = (uint32_t)acc;  // in (0, 11p): non-negative, below 2^31
+        fp_cond_sub(t, P8L);
+        fp_cond_sub(t, P4L);
+        fp_cond_sub(t, P2L);
+#pragma unroll
+        for (int i = 0; i < NL; ++i) r.v[i] = t[i];
+    }
+    __device__ static __forceinline__ void sqr_sub2z(F& r, const F& a, const F& c) {
+        Fp z;
+        fp_set_zero(z);
+        sqr_sub2(r, a, z, c);
     }
     __device__ static __forceinline__ bool is_zero(const F& a) {
         const uint32_t z = fp_is_zero(a) ? 1u : 0u;

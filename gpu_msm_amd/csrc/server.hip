@@ -108,17 +108,18 @@ struct HostMlp {
             for (int i = 0; i < 32; ++i) a = fmaf(W3[k * 32 + i], h2[i], a);
             y[k] = a;
         }
-        const double cc = y[0] * 4.0 + 12.0, ls = y[1] * 1.5 + 3.0;
-        if (!std::isfinite(cc) || !std::isfinite(ls) || cc < 2.0 || cc > 21.0) return;  // heuristic fallback
+        // outputs: a correction to the heuristic's width, and log2 seg
+        const double dc = y[0] * 2.0, ls = y[1] * 1.5 + 3.0;
+        if (!std::isfinite(dc) || !std::isfinite(ls) || std::fabs(dc) > 20.0) return;  // heuristic fallback
         // c inside the sweeps' candidate band around the heuristic, at the
         // nearest width whose top window is not sparse
         const int h = gm::choose_window(curve, std::max(n, 1), bits);
         const int lo = std::max(3, h - 3), hi = std::min(20, h + 2);
-        const double ct = std::min((double)hi, std::max((double)lo, cc));
+        const double ct = std::min((double)hi, std::max((double)lo, h + dc));
         c = h;
         for (int k = lo; k <= hi; ++k)
             if (gm::window_ok(bits, k) && std::fabs(k - ct) < std::fabs(c - ct)) c = k;
-        seg = 1 << (int)lround(std::min(5.0, std::max(1.0, ls)));
+        seg = 1 << (int)lround(std::min(5.0, std::max(2.0, ls)));
         seg = std::min(seg, 1 << (c - 1));
     }
 };

@@ -9,7 +9,8 @@ segs)``:
   batch size), nearest neighbour in (log2 N, log2 batch).
 - ``MlpPlanner``: an FP32 MLP (inputs log2 N, batch size, curve; two hidden
   layers of 32) trained on the autotuner's sweeps by the library's own HIP
-  kernels. Its outputs are rounded to the nearest valid (c, seg): c is kept
+  kernels. It predicts a correction to the heuristic's width and the
+  bucket split; the width h + correction is rounded to the nearest valid c
   inside the sweeps' candidate band (heuristic c - 3 .. + 2, where the
   network was trained) and off widths with a sparse top window; a non-finite
   prediction falls back to the heuristic.
@@ -41,29 +42,30 @@ def features(log2n, batch, curve) -> np.ndarray:
     return np.stack([(log2n - 13.0) / 5.0, (lb - 3.0) / 3.0, cv], axis=-1).astype(np.float32)
 
 
-def targets(c, seg) -> np.ndarray:
+def targets(c, seg, h) -> np.ndarray:
+    """MLP outputs for a plan: the width as a correction to the heuristic's
+    width h, (c - h)/2, and (log2 seg - 3)/1.5. The network learns where the
+    analytic model is off (notably with the batch size, which it does not
+    see), not the width itself: trained on all but a fifth of the sweep's
+    cells, absolute targets gain nothing on the held-out fifth, corrections do
+    (tests/test_planner_gain_cpu.py)."""
     c = np.asarray(c, dtype=np.float64)
+    h = np.asarray(h, dtype=np.float64)
     ls = np.log2(np.asarray(seg, dtype=np.float64))
-    return np.stack([(c - 12.0) / 4.0, (ls - 3.0) / 1.5], axis=-1).astype(np.float32)
+    return np.stack([(c - h) / 2.0, (ls - 3.0) / 1.5], axis=-1).astype(np.float32)
 
 
 def decode(y: np.ndarray):
-    """MLP outputs -> (c, seg) rounded to valid values, or None if unusable."""
-    c = y[..., 0] * 4.0 + 12.0
+    """MLP outputs -> (width correction dc, seg), or None if unusable."""
+    dc = y[..., 0] * 2.0
     ls = y[..., 1] * 1.5 + 3.0
     out = []
-    for ci, li in zip(np.atleast_1d(c), np.atleast_1d(ls)):
-        if not (math.isfinite(ci) and math.isfinite(li)):
+    for di, li in zip(np.atleast_1d(dc), np.atleast_1d(ls)):
+        if not (math.isfinite(di) and math.isfinite(li)) or abs(di) > C_MAX:
             out.append(None)
             continue
-        cc = int(round(ci))
-        if cc < C_MIN - 1 or cc > C_MAX + 1:
-            out.append(None)
-            continue
-        cc = min(max(cc, C_MIN), C_MAX)
         seg = 2 ** int(round(min(max(li, 2.0), 5.0)))
-        seg = min(seg, 1 << (cc - 1))
-        out.append((cc, seg))
+        out.append((float(di), seg))
     return out
 
 
@@ -169,7 +171,7 @@ class MlpPlanner:
             n = int(sizes[k])
             if n not in heur:
                 heur[n] = gm.plan_query(curve, max(n, 1), 5, 299)[0]
-            c = snap_window(d[0], heur[n])
+            c = snap_window(heur[n] + d[0], heur[n])
             w[k], s[k] = c, min(d[1], 1 << (c - 1))
         return w, s
 

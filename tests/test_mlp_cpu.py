@@ -2,6 +2,7 @@
 differences, and the planner's feature/decode helpers. CPU only."""
 import numpy as np
 
+import gpu_msm_amd as gm
 from gpu_msm_amd import planner as pl
 from oracle import mlp_oracle as mo
 
@@ -37,11 +38,15 @@ def test_oracle_adam_reduces_loss():
 def test_planner_features_and_decode_roundtrip():
     x = pl.features(20, 1, 0)
     assert np.allclose(x, [1.4, -1.0, 0.0])
-    y = pl.targets([16, 9, 4], [8, 4, 32])
+    y = pl.targets([16, 9, 4], [8, 4, 32], [15, 10, 4])
     dec = pl.decode(y)
-    assert dec == [(16, 8), (9, 4), (4, 8)]  # seg clamped to 2^(c-1)
+    assert dec == [(1.0, 8), (-1.0, 4), (0.0, 32)]  # width corrections to the heuristic's
     assert pl.decode(np.array([[np.nan, 0.0]], dtype=np.float32)) == [None]
-    assert pl.decode(np.array([[10.0, 0.0]], dtype=np.float32)) == [None]  # c = 52: out of range
+    assert pl.decode(np.array([[11.0, 0.0]], dtype=np.float32)) == [None]  # correction of 22: out of range
+    mlp = pl.MlpPlanner(np.zeros(1250, dtype=np.float32))
+    h = gm.plan_query(1, 1 << 12, 5, 299)[0]
+    w, s = mlp.plans_from_outputs(1, np.array([1 << 12]), pl.targets([h + 2], [8], [h]))
+    assert (w[0] == h + 2 or not pl.window_ok(h + 2)) and s[0] == 8
 
 
 def test_table_planner_nearest_cell():
