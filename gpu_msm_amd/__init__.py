@@ -549,7 +549,7 @@ def last_msm_info(gpu_index: int = 0) -> dict:
 class _ServerConfig(ctypes.Structure):
     _fields_ = [("gpu_index", ctypes.c_uint32), ("slots", ctypes.c_int), ("max_batch", ctypes.c_int),
                 ("hbm_budget_bytes", ctypes.c_size_t), ("planner", ctypes.c_int),
-                ("mlp_params", ctypes.c_void_p)]
+                ("mlp_params", ctypes.c_void_p), ("small_max_n", ctypes.c_int)]
 
 
 class _ServerStats(ctypes.Structure):
@@ -570,11 +570,14 @@ class Server:
 
     def __init__(self, g1_pool: Bases | None, g2_pool: Bases | None, slots: int = 3, max_batch: int = 64,
                  hbm_budget_bytes: int = 32 << 30, planner: int = PLANNER_HEURISTIC, mlp_params=None,
-                 gpu_index: int = 0):
+                 gpu_index: int = 0, small_max_n: int = 0):
+        """small_max_n: requests of at most this many points form their own
+        batches, go first and keep one slot to themselves (0: 2^16, < 0: no
+        small class)."""
         self.pools = (g1_pool, g2_pool)
         self._mlp = None if mlp_params is None else np.ascontiguousarray(mlp_params, dtype=np.float32)
         cfg = _ServerConfig(gpu_index, slots, max_batch, hbm_budget_bytes, planner,
-                            None if self._mlp is None else self._mlp.ctypes.data)
+                            None if self._mlp is None else self._mlp.ctypes.data, small_max_n)
         h = ctypes.c_void_p()
         _check(lib().gm_server_create(ctypes.byref(h), ctypes.byref(cfg), g1_pool.h if g1_pool else None,
                                       g2_pool.h if g2_pool else None), "server create")

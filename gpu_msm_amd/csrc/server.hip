@@ -2,15 +2,22 @@
 // bases resident in HBM.
 //
 // Requests (curve, base offset, n, scalars) are queued by gm_server_submit.
-// A scheduler thread forms a batch from the head of the queue (one curve and
-// scalar width per batch, FIFO order) as soon as a stream slot is free,
-// admitting requests while the batch's device footprint (pipeline workspace +
-// scalars) fits what is left of the HBM budget, then launches the whole
-// pipeline on that slot's stream. Several slots run at once, so batches that
-// form while others are on the GPU overlap on the device. A completion thread
-// waits for batches in launch order, runs the window combines on host threads
-// while the next batches keep the GPU busy, and publishes each request's
-// affine result and latency (submit -> result ready).
+// A scheduler thread forms a batch as soon as a stream slot is free, admitting
+// requests while the batch's device footprint (pipeline workspace + scalars)
+// fits what is left of the HBM budget, then launches the whole pipeline on
+// that slot's stream. Several slots run at once, so batches that form while
+// others are on the GPU overlap on the device.
+//
+// Requests of at most small_max_n points form their own batches and go first
+// (FIFO among them), and one slot is kept for them: batches of larger
+// requests occupy at most slots - 1 slots, so a small request never queues
+// behind large MSMs (a 2^20 G2 MSM holds its slot for some 16 ms). Every
+// batch holds one curve, scalar width and size class.
+//
+// One completion thread per slot waits for that slot's batch, runs its window
+// combines on host threads and publishes each request's affine result and
+// latency (submit -> result ready), so batches complete in whatever order
+// they finish, not in launch order.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <string.h>
@@ -69,6 +76,8 @@ struct Slot {
     uint8_t* pinned = nullptr;
     size_t pinned_cap = 0;
     bool busy = false;
+    bool launched = false;  // device work queued; the slot's completion thread owns it
+    bool large = false;     // a batch of the large size class
     bool fail = false;
     std::vector<Request*> reqs;
     std::vector<gm::MsmDesc> desc;
@@ -135,8 +144,8 @@ struct gm_server {
     std::deque<Request*> queue;
     std::unordered_map<uint64_t, Request*> table;
     std::vector<Slot> slots;
-    std::deque<int> launched;  // slot ids in launch order
     size_t inflight_bytes = 0;
+    int small_max_n = 1 << 16;  // largest request of the small class; 0: no small lane
     uint64_t next_ticket = 1;
     bool stop = false;
     int fail_batches = 0;
@@ -146,7 +155,8 @@ struct gm_server {
     size_t peak_inflight = 0;
     int peak_busy = 0;
     std::vector<double> lat;
-    std::thread th_sched, th_comp;
+    std::thread th_sched;
+    std::vector<std::thread> th_comp;  // one per slot
 };
 
 namespace {
@@ -206,37 +216,67 @@ hipError_t launch_slot(gm_server* S, Slot& sl) {
     return hipEventRecord(sl.ev, sl.stream);
 }
 
+bool is_large(const gm_server* S, const Request* r) { return S->small_max_n > 0 && r->n > S->small_max_n; }
+
+// The request that heads the next batch and a free slot for it, or null:
+// small requests first (FIFO among them); large ones while fewer than
+// slots - 1 large batches are in flight (one slot stays with the small
+// class; with one slot, or no small lane, every slot takes either).
+const Request* pick_head(gm_server* S, int& sid) {
+    sid = -1;
+    int free_slot = -1, large_busy = 0;
+    for (size_t i = 0; i < S->slots.size(); ++i) {
+        if (!S->slots[i].busy) {
+            if (free_slot < 0) free_slot = (int)i;
+        } else if (S->slots[i].large) {
+            ++large_busy;
+        }
+    }
+    if (free_slot < 0) return nullptr;
+    const Request* small = nullptr;
+    const Request* big = nullptr;
+    for (const Request* r : S->queue) {
+        if (!is_large(S, r)) {
+            small = r;
+            break;
+        }
+        if (!big) big = r;
+    }
+    const int nslots = (int)S->slots.size();
+    const int large_cap = (S->small_max_n > 0 && nslots > 1) ? nslots - 1 : nslots;
+    const Request* head = small ? small : (big && large_busy < large_cap ? big : nullptr);
+    if (head) sid = free_slot;
+    return head;
+}
+
 void scheduler(gm_server* S) {
     hipSetDevice((int)S->cfg.gpu_index);
     for (;;) {
         std::unique_lock<std::mutex> lk(S->mu);
         int sid = -1;
+        const Request* head = nullptr;
         S->cv_sched.wait(lk, [&] {
             if (S->stop) return true;
-            if (S->queue.empty()) return false;
-            for (size_t i = 0; i < S->slots.size(); ++i)
-                if (!S->slots[i].busy) return true;
-            return false;
+            head = pick_head(S, sid);
+            return head != nullptr;
         });
         if (S->stop) return;
-        for (size_t i = 0; i < S->slots.size(); ++i)
-            if (!S->slots[i].busy) {
-                sid = (int)i;
-                break;
-            }
-        // admission: FIFO over one (curve, limbs, bits) class, by footprint
-        // (estimated plans during admission, the exact plan once at the end)
+        // admission: FIFO over one (curve, limbs, bits, size) class, by
+        // footprint (estimated plans during admission, the exact plan once at
+        // the end)
         std::vector<Request*> take;
         gm::Plan p{};
         std::vector<gm::MsmDesc> desc;
-        const Request* head = S->queue.front();
+        const bool large = is_large(S, head);
         const size_t budget = S->cfg.hbm_budget_bytes;
         std::vector<size_t> qidx;
         Request* reject = nullptr;
         size_t reject_qi = 0;
         for (size_t qi = 0; qi < S->queue.size() && (int)take.size() < S->cfg.max_batch; ++qi) {
             Request* r = S->queue[qi];
-            if (r->curve != head->curve || r->limbs != head->limbs || r->bits != head->bits) continue;
+            if (r->curve != head->curve || r->limbs != head->limbs || r->bits != head->bits ||
+                is_large(S, r) != large)
+                continue;
             take.push_back(r);
             gm::Plan tp{};
             std::vector<gm::MsmDesc> td;
@@ -281,6 +321,8 @@ void scheduler(gm_server* S) {
         for (auto it = qidx.rbegin(); it != qidx.rend(); ++it) S->queue.erase(S->queue.begin() + (long)*it);
         Slot& sl = S->slots[sid];
         sl.busy = true;
+        sl.launched = false;
+        sl.large = large;
         sl.reqs = take;
         sl.p = p;
         sl.desc = desc;
@@ -320,24 +362,23 @@ void scheduler(gm_server* S) {
             if (hipEventRecord(sl.ev, sl.stream) != hipSuccess) (void)hipStreamSynchronize(sl.stream);
         }
         lk.lock();
-        S->launched.push_back(sid);
+        sl.launched = true;
         S->batches += 1;
         S->batched_msms += take.size();
-        S->cv_comp.notify_one();
+        S->cv_comp.notify_all();
     }
 }
 
-void completer(gm_server* S) {
+// Completion of one slot's batches, in whatever order the slots finish.
+void completer(gm_server* S, int sid) {
     hipSetDevice((int)S->cfg.gpu_index);
+    Slot& sl = S->slots[sid];
     for (;;) {
-        int sid;
         {
             std::unique_lock<std::mutex> lk(S->mu);
-            S->cv_comp.wait(lk, [&] { return S->stop || !S->launched.empty(); });
-            if (S->launched.empty()) return;  // stop with nothing in flight
-            sid = S->launched.front();
+            S->cv_comp.wait(lk, [&] { return S->stop || (sl.busy && sl.launched); });
+            if (!(sl.busy && sl.launched)) return;  // stop with nothing in flight on this slot
         }
-        Slot& sl = S->slots[sid];
         hipError_t e = hipEventSynchronize(sl.ev);
         int rc = 0;
         if (sl.fail || e != hipSuccess) {
@@ -393,8 +434,8 @@ void completer(gm_server* S) {
         }
         sl.reqs.clear();
         sl.busy = false;
+        sl.launched = false;
         S->inflight_bytes -= sl.footprint;
-        S->launched.pop_front();
         S->cv_done.notify_all();
         S->cv_sched.notify_one();
     }
@@ -437,8 +478,12 @@ int gm_server_create(gm_server** out, const gm_server_config* cfg, const gm_base
             return -2;
         }
     }
+    if (cfg->small_max_n > 0)
+        S->small_max_n = cfg->small_max_n;
+    else if (cfg->small_max_n < 0)
+        S->small_max_n = 0;
     S->th_sched = std::thread(scheduler, S);
-    S->th_comp = std::thread(completer, S);
+    for (int i = 0; i < cfg->slots; ++i) S->th_comp.emplace_back(completer, S, i);
     *out = S;
     return 0;
 }
@@ -523,13 +568,19 @@ void gm_server_destroy(gm_server* S) {
     {
         std::unique_lock<std::mutex> lk(S->mu);
         // drain: wait until queued and in-flight work has completed
-        S->cv_done.wait(lk, [&] { return S->queue.empty() && S->launched.empty(); });
+        S->cv_done.wait(lk, [&] {
+            if (!S->queue.empty()) return false;
+            for (auto& sl : S->slots)
+                if (sl.busy) return false;
+            return true;
+        });
         S->stop = true;
         S->cv_sched.notify_all();
         S->cv_comp.notify_all();
     }
     if (S->th_sched.joinable()) S->th_sched.join();
-    if (S->th_comp.joinable()) S->th_comp.join();
+    for (auto& th : S->th_comp)
+        if (th.joinable()) th.join();
     hipSetDevice((int)S->cfg.gpu_index);
     for (auto& sl : S->slots) {
         if (sl.stream) hipStreamSynchronize(sl.stream);

@@ -57,7 +57,7 @@ class Stream:
 
 def run(stream: Stream, rate: float, count: int, lo: int, hi: int, seed: int = 5, slots: int = 3,
         max_batch: int = 64, budget: int = 48 << 30, planner: int = gm.PLANNER_HEURISTIC, mlp_params=None,
-        check: int = 16, log=None, results_out: list | None = None) -> dict:
+        check: int = 16, log=None, results_out: list | None = None, small_max_n: int = 0) -> dict:
     """Serves `count` requests; with results_out, appends (curve, n,
     base_offset, scalars, rc, affine record) per request for the caller to
     check."""
@@ -65,7 +65,8 @@ def run(stream: Stream, rate: float, count: int, lo: int, hi: int, seed: int = 5
     arr = np.cumsum(rng.exponential(1.0 / rate, size=count))  # scheduled arrivals (s)
     scal = [stream.scalar_slice(rng, int(n)) for n in sizes]
     srv = gm.Server(stream.g1, stream.g2, slots=slots, max_batch=max_batch, hbm_budget_bytes=budget,
-                    planner=planner, mlp_params=mlp_params)
+                    planner=planner, mlp_params=mlp_params, small_max_n=small_max_n)
+    server_lat = np.zeros(count)
     tickets = [None] * count
     ready = [threading.Event() for _ in range(count)]
     results = [None] * count
@@ -83,8 +84,9 @@ def run(stream: Stream, rate: float, count: int, lo: int, hi: int, seed: int = 5
     def waiter():
         for i in range(count):
             ready[i].wait()
-            rc, res, _ = srv.wait(tickets[i], int(curves[i]))
+            rc, res, lat = srv.wait(tickets[i], int(curves[i]))
             client_lat[i] = (time.perf_counter() - (t0 + arr[i])) * 1e3
+            server_lat[i] = lat
             results[i] = (rc, res)
 
     ts, tw = threading.Thread(target=submitter), threading.Thread(target=waiter)
@@ -111,8 +113,20 @@ def run(stream: Stream, rate: float, count: int, lo: int, hi: int, seed: int = 5
         alone = gm.msm_resident(stream.pool(cv), sc, n=n, base_offset=b, scalar_bits=299)
         sc.free()
         same &= results[i][0] == 0 and alone.tobytes() == results[i][1].tobytes()
+    # latency per curve and size class (the server's small class: N <= 2^16)
+    small_n = (1 << 16) if small_max_n == 0 else small_max_n
+    by_class = {}
+    for cv, cname in ((gm.CURVE_G1, "g1"), (gm.CURVE_G2, "g2")):
+        for small, sname in ((True, f"n_le_{small_n}"), (False, f"n_gt_{small_n}")):
+            sel = (curves == cv) & ((sizes <= small_n) == small)
+            if sel.any():
+                v = server_lat[sel]
+                by_class[f"{cname}_{sname}"] = {"requests": int(sel.sum()),
+                                                "p50_ms": round(float(np.percentile(v, 50)), 3),
+                                                "p99_ms": round(float(np.percentile(v, 99)), 3)}
     return {
         "requests": count, "rate_per_s": rate, "completed_ok": ok, "wall_s": round(wall, 3),
+        "latency_by_class": by_class,
         "msms_per_s": round(ok / wall, 1),
         "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3), "max_ms": round(st["max_ms"], 3),
         "client_p50_ms": round(float(np.percentile(client_lat, 50)), 3),

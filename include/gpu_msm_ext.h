@@ -136,6 +136,8 @@ typedef struct gm_server_config {
     size_t hbm_budget_bytes;  /* admission limit for in-flight footprints */
     int planner;              /* GM_PLANNER_* */
     const float* mlp_params;  /* GM_MLP_PARAMS floats when planner == MLP */
+    int small_max_n;          /* requests of at most this many points form their own batches, go first and keep one
+                                 slot to themselves; 0: 2^16, < 0: no small class */
 } gm_server_config;
 
 typedef struct gm_server_stats {

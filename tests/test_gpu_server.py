@@ -125,3 +125,32 @@ def test_scalar_bound_error_and_bad_arguments(pools):
     with pytest.raises(gm.MsmError):
         srv.submit(gm.CURVE_G1, len(ks) - 2, big, scalar_bits=299)  # past the end of the pool
     srv.close()
+
+
+def test_small_requests_pass_large_ones(pools):
+    """Size classes: with small_max_n = 256 and two slots, large requests
+    (G2, 2^14 points, one per batch) hold at most one slot at a time, so a
+    small request submitted behind four of them takes the other slot at once
+    and completes before the last large one, whose batch waits for the three
+    before it; every result equals the same MSM alone."""
+    ks, g1, g2 = pools
+    n_big = 1 << 14
+    big = [o.gen_scalars_modr(500 + k, n_big) for k in range(4)]
+    small = o.gen_scalars_modr(510, 64)
+    srv = gm.Server(g1, g2, slots=2, max_batch=1, small_max_n=256)
+    tb = [srv.submit(gm.CURVE_G2, 0, sc, scalar_bits=299) for sc in big]
+    ts = srv.submit(gm.CURVE_G1, 100, small, scalar_bits=299)
+    outs_b = [srv.wait(t, gm.CURVE_G2) for t in tb]
+    rc_s, res_s, lat_s = srv.wait(ts, gm.CURVE_G1)
+    st = srv.stats()
+    srv.close()
+    assert rc_s == 0 and all(rc == 0 for rc, _, _ in outs_b)
+    print(f"small request {lat_s:.2f} ms; large requests {[round(l, 2) for _, _, l in outs_b]} ms", flush=True)
+    assert lat_s < outs_b[-1][2]
+    assert st["peak_concurrent_batches"] <= 2
+    s = gm.Scalars.upload(small)
+    assert gm.msm_resident(g1, s, n=64, base_offset=100, scalar_bits=299).tobytes() == res_s.tobytes()
+    want = o.msm("g1", o.bases_from_k("g1", ks[100:164]), small)
+    assert want.tobytes() == res_s.tobytes()
+    s2 = gm.Scalars.upload(big[3])
+    assert gm.msm_resident(g2, s2, n=n_big, base_offset=0, scalar_bits=299).tobytes() == outs_b[3][1].tobytes()
