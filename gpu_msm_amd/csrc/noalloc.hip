@@ -23,7 +23,7 @@
 
 namespace {
 
-// 96 MiB: room for G1 MSMs up to 2^15 points and G2 up to 2^14 with 5-limb
+// 96 MiB: room for G1 MSMs up to 2^16 points and G2 up to 2^15 with 5-limb
 // scalars (the limit is the plan's exact workspace plus the imported bases).
 constexpr size_t SCRATCH_BYTES = (size_t)96 << 20;
 static __device__ __align__(256) uint8_t scratch[SCRATCH_BYTES];

@@ -177,7 +177,7 @@ void gm_server_destroy(gm_server* s);
  * for the null stream) with no host wait. It calls no hipMalloc / hipFree or
  * allocating library routine: the workspace is one fixed 96 MiB device array
  * per GPU, held until the process ends, and the window combine runs on the
- * device. Sizes whose workspace does not fit (about 2^15 G1 or 2^14 G2 points
+ * device. Sizes whose workspace does not fit (about 2^16 G1 or 2^15 G2 points
  * with 5-limb scalars) return -6. Calls that overlap on one GPU share that
  * array and race: serialise them. Returns 0; -1 bad arguments; -2 device or
  * set-up failure; -4 launch failure; -6 too large for the fixed workspace. */
