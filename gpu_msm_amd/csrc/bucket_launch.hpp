@@ -1,9 +1,10 @@
 // Launchers of the bucket kernels (accumulation, fix-up, bucket reduction,
 // window sums), shared by the per-curve translation units. O is the storage
 // policy; the kernels run with KernelOps<O> (Fp2LaneOps for G2: two lanes per
-// item, so every launch has Stor<K>::LPI threads per item); a plan with
-// tail_split runs the bucket reduction and window sums with SplitOps<O>
-// (FpPairOps for G1, Fp2QuadOps for G2).
+// item, so every launch has Stor<K>::LPI threads per item), the accumulation
+// with AccOps<O> (for G2 the same lane pairs, single-chain column sums); a
+// plan with tail_split runs the bucket reduction and window sums with
+// SplitOps<O> (FpPairOps for G1, Fp2QuadOps for G2).
 #pragma once
 #include "engine_impl.hpp"
 #include "msm_kernels.hpp"
@@ -24,7 +25,7 @@ struct SplitOps<Fp2Ops> {
 
 template <class O>
 hipError_t launch_accumulate(hipStream_t s, const Plan& p, const Aff<O>* d_bases, uint8_t* ws) {
-    using K = typename KernelOps<O>::type;
+    using K = typename AccOps<O>::type;
     using X = typename Stor<K>::X;
     if (p.lanes <= 0) return hipSuccess;
     k_accumulate<K><<<blocks_for((long)p.lanes * Stor<K>::LPI, 64), 64, 0, s>>>(

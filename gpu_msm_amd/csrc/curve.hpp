@@ -46,6 +46,10 @@ template <>
 __device__ __forceinline__ bool aff_is_inf<Fp2LaneOps>(const Aff<Fp2LaneOps>& p) {
     return p.x.v[NL - 1] == INF_MARK;
 }
+template <>
+__device__ __forceinline__ bool aff_is_inf<Fp2LaneAccOps>(const Aff<Fp2LaneAccOps>& p) {
+    return p.x.v[NL - 1] == INF_MARK;
+}
 
 template <class O>
 __device__ __forceinline__ void xyzz_set_inf(Xyzz<O>& p) {

@@ -248,6 +248,21 @@ constexpr uint32_t HALFL[NL] = {0x5585555, 0x8e00135, 0x9dae618, 0x229462b, 0x6f
 // a^(p-2) (Fermat inverse); 0 maps to 0.
 __device__ inline void fp_inv(Fp& r, const Fp& a) { fp_pow_words(r, a, PM2_WORDS, 14); }
 
+// Column accumulation acc += x * y as one signed (x, y) or unsigned (m, c)
+// mad on the running column sum itself. In plain C++ the compiler sums each
+// column's products on a chain of its own and adds it to the running value
+// afterwards (one more 64-bit add per column); these keep the whole product
+// on one chain. The lane-pair accumulation kernel takes them (CHAIN = true);
+// the depth-bound kernels, whose few waves need the compiler's independent
+// chains to hide the mad latency, keep plain C++. The mad's carry-out is
+// unused (vcc).
+__device__ __forceinline__ void mac_i(int64_t& acc, int32_t x, int32_t y) {
+    asm("v_mad_i64_i32 %0, vcc, %1, %2, %0" : "+v"(acc) : "v"(x), "v"(y) : "vcc");
+}
+__device__ __forceinline__ void mac_ic(int64_t& acc, uint32_t m, uint32_t c) {
+    asm("v_mad_u64_u32 %0, vcc, %1, %2, %0" : "+v"(acc) : "v"(m), "s"(c) : "vcc");
+}
+
 // Signed Montgomery reduction of a sum of limb products, the core of the
 // lazily reduced products below: value = sum over the T terms of x_t * y_t,
 // with y limbs signed (a negated operand makes a difference of products).
@@ -256,21 +271,39 @@ __device__ inline void fp_inv(Fp& r, const Fp& a) { fp_pow_words(r, a, PM2_WORDS
 // 16 reduction products stay below 2^63. Returns the reduced value in
 // 28-bit limbs with its top limb signed (in the last word, sign carried by
 // the int64 returned as `top`).
-template <int T>
+template <int T, bool CHAIN = false>
 __host__ __device__ __forceinline__ void redc_signed(uint32_t (&t)[NL], int64_t& top, const Fp* const (&x)[T],
                                             const int32_t (*const (&y)[T])[NL]) {
     uint32_t m[NL];
     int64_t acc = 0;
+    auto mx = [&](int32_t a, int32_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+        if constexpr (CHAIN) {
+            mac_i(acc, a, b);
+            return;
+        }
+#endif
+        acc += (int64_t)a * b;
+    };
+    auto mp = [&](uint32_t a, uint32_t c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+        if constexpr (CHAIN) {
+            mac_ic(acc, a, c);
+            return;
+        }
+#endif
+        acc += (int64_t)((uint64_t)a * c);
+    };
 #pragma unroll
     for (int k = 0; k < NL; ++k) {
 #pragma unroll
         for (int i = 0; i <= k; ++i)
 #pragma unroll
-            for (int u = 0; u < T; ++u) acc += (int64_t)(int32_t)x[u]->v[i] * (*y[u])[k - i];
+            for (int u = 0; u < T; ++u) mx((int32_t)x[u]->v[i], (*y[u])[k - i]);
 #pragma unroll
-        for (int i = 0; i < k; ++i) acc += (int64_t)((uint64_t)m[i] * PL[k - i]);
+        for (int i = 0; i < k; ++i) mp(m[i], PL[k - i]);
         m[k] = ((uint32_t)acc * PINV28) & MASK28;
-        acc += (int64_t)((uint64_t)m[k] * PL[0]);
+        mp(m[k], PL[0]);
         acc >>= 28;  // arithmetic: the low 28 bits are zero
     }
 #pragma unroll
@@ -278,8 +311,8 @@ __host__ __device__ __forceinline__ void redc_signed(uint32_t (&t)[NL], int64_t&
 #pragma unroll
         for (int i = k - NL + 1; i < NL; ++i) {
 #pragma unroll
-            for (int u = 0; u < T; ++u) acc += (int64_t)(int32_t)x[u]->v[i] * (*y[u])[k - i];
-            acc += (int64_t)((uint64_t)m[i] * PL[k - i]);
+            for (int u = 0; u < T; ++u) mx((int32_t)x[u]->v[i], (*y[u])[k - i]);
+            mp(m[i], PL[k - i]);
         }
         t[k - NL] = (uint32_t)acc & MASK28;
         acc >>= 28;
@@ -754,6 +787,7 @@ __device__ __forceinline__ void lane_operands(int32_t (&y)[NL], int32_t (&w)[NL]
 }
 
 // Own component of a b; ao / bo are the partner's parts.
+template <bool CHAIN = false>
 __device__ __forceinline__ void fp2_lane_mul(Fp& r, const Fp& a, const Fp& ao, const Fp& b, const Fp& bo) {
     int32_t y[NL], w[NL];
     lane_operands(y, w, b, bo, false);
@@ -761,7 +795,7 @@ __device__ __forceinline__ void fp2_lane_mul(Fp& r, const Fp& a, const Fp& ao, c
     const int32_t(*const yy[2])[NL] = {&y, &w};
     uint32_t t[NL];
     int64_t top;
-    redc_signed<2>(t, top, x, yy);
+    redc_signed<2, CHAIN>(t, top, x, yy);
     // part 0 lies in (-p, 2p), part 1 in [0, 3p)
     redc_fold(r, t, top, true);
 }
@@ -770,6 +804,7 @@ __device__ __forceinline__ void fp2_lane_mul(Fp& r, const Fp& a, const Fp& ao, c
 // [0, 2p) both parts lie in (-8p^2, 8p^2) before and in (-2p, 3p) after the
 // reduction; with a and b raw (magnitude below 2p) in (-16p^2, 16p^2) and
 // (-4p, 5p), which `wide` folds.
+template <bool CHAIN = false>
 __device__ __forceinline__ void fp2_lane_mul_sub(Fp& r, const Fp& a, const Fp& ao, const Fp& b, const Fp& bo,
                                                  const Fp& c, const Fp& co, const Fp& d, const Fp& dox,
                                                  bool wide = false) {
@@ -780,7 +815,7 @@ __device__ __forceinline__ void fp2_lane_mul_sub(Fp& r, const Fp& a, const Fp& a
     const int32_t(*const yy[4])[NL] = {&y0, &w0, &y1, &w1};
     uint32_t t[NL];
     int64_t top;
-    redc_signed<4>(t, top, x, yy);
+    redc_signed<4, CHAIN>(t, top, x, yy);
     if (wide)
         redc_fold_wide(r, t, top);
     else
@@ -792,6 +827,7 @@ __device__ __forceinline__ void fp2_lane_mul_sub(Fp& r, const Fp& a, const Fp& a
 // (-4p^2, 4p^2), part 1 (a1 + a1) a0 in (-8p^2, 8p^2), one signed product per
 // lane (limb products below 2^58, a column below 2^63). The reduced value
 // lies in (-2p, 3p); returned as t / top before the fold.
+template <bool CHAIN = false>
 __device__ __forceinline__ void fp2_lane_sqr_cols(uint32_t (&t)[NL], int64_t& top, const Fp& a) {
     Fp ao, x;
     fp_pair_swap(ao, a);
@@ -805,10 +841,13 @@ __device__ __forceinline__ void fp2_lane_sqr_cols(uint32_t (&t)[NL], int64_t& to
     }
     const Fp* const xs[1] = {&x};
     const int32_t(*const ys[1])[NL] = {&y};
-    redc_signed<1>(t, top, xs, ys);
+    redc_signed<1, CHAIN>(t, top, xs, ys);
 }
 
-struct Fp2LaneOps {
+// CHAIN selects the single-chain column sums of redc_signed (the
+// accumulation kernel's policy, Fp2LaneAccOps).
+template <bool CHAIN>
+struct Fp2LaneOpsT {
     using F = Fp;
     static constexpr int WORDS = NL;
     static constexpr bool RAW = true;  // has sqr_sub2 / sqr_sub2z
@@ -820,7 +859,7 @@ struct Fp2LaneOps {
         Fp ao, bo;
         fp_pair_swap(ao, a);
         fp_pair_swap(bo, b);
-        fp2_lane_mul(r, a, ao, b, bo);
+        fp2_lane_mul<CHAIN>(r, a, ao, b, bo);
     }
     // (a0 + a1 i)^2 = (a0 + a1)(a0 - a1) + 2 a0 a1 i: one plain product per
     // lane (part 0: (a0 + a1)(a0 - a1), part 1: (a1 + a1) a0), operands
@@ -843,7 +882,7 @@ struct Fp2LaneOps {
         fp_pair_swap(bo, b);
         fp_pair_swap(co, c);
         fp_pair_swap(dx, d);
-        fp2_lane_mul_sub(r, a, ao, b, bo, c, co, d, dx);
+        fp2_lane_mul_sub<CHAIN>(r, a, ao, b, bo, c, co, d, dx);
     }
     // Raw differences, as FpOps has them: a difference that only feeds
     // products stays limb by limb (signed limbs, value of magnitude below
@@ -854,7 +893,7 @@ struct Fp2LaneOps {
     __device__ static __forceinline__ void sqr_raw(F& r, const F& a) {
         uint32_t t[NL];
         int64_t top;
-        fp2_lane_sqr_cols(t, top, a);
+        fp2_lane_sqr_cols<CHAIN>(t, top, a);
         redc_fold(r, t, top, true);
     }
     __device__ static __forceinline__ void mul_raw(F& r, const F& a, const F& b) { mul(r, a, b); }
@@ -864,7 +903,7 @@ struct Fp2LaneOps {
         fp_pair_swap(bo, b);
         fp_pair_swap(co, c);
         fp_pair_swap(dx, d);
-        fp2_lane_mul_sub(r, a, ao, b, bo, c, co, d, dx, true);
+        fp2_lane_mul_sub<CHAIN>(r, a, ao, b, bo, c, co, d, dx, true);
     }
     // Own component of a^2 - b - 2c (a raw, b and c in [0, 2p)): the square
     // reduces to (-2p, 3p); 8p - b - 2c (limbs 8 p_i - b_i - 2 c_i, in
@@ -874,7 +913,7 @@ struct Fp2LaneOps {
     __device__ static __forceinline__ void sqr_sub2(F& r, const F& a, const F& b, const F& c) {
         uint32_t t[NL];
         int64_t top;
-        fp2_lane_sqr_cols(t, top, a);
+        fp2_lane_sqr_cols<CHAIN>(t, top, a);
         int64_t acc = 0;
 #pragma unroll
         for (int i = 0; i < NL - 1; ++i) {
@@ -922,6 +961,8 @@ struct Fp2LaneOps {
             fp_set_one(r);
     }
 };
+using Fp2LaneOps = Fp2LaneOpsT<false>;
+using Fp2LaneAccOps = Fp2LaneOpsT<true>;
 
 
 // G1 policy of the depth-bound tail kernels (bucket reduction, window sums):

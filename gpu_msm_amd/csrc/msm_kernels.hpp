@@ -120,6 +120,8 @@ struct Stor<Fp2LaneOps> {
     using X = Xyzz<Fp2Ops>;
     static constexpr int LPI = 2;
 };
+template <>
+struct Stor<Fp2LaneAccOps> : Stor<Fp2LaneOps> {};
 // Tail kernels on split lane groups: G1 two lanes per item, each holding the
 // whole record; G2 four (two lane pairs), each pair holding the record.
 template <>
@@ -147,12 +149,14 @@ template <class O>
 __device__ __forceinline__ void stp(Xyzz<O>* d, const Xyzz<O>& v) {
     st16(d, v);
 }
-__device__ __forceinline__ void ldp(Aff<Fp2LaneOps>& d, const Aff<Fp2Ops>* s) {
+template <bool C>
+__device__ __forceinline__ void ldp(Aff<Fp2LaneOpsT<C>>& d, const Aff<Fp2Ops>* s) {
     const uint32_t q = lane_part();
     ld16(d.x, &s->x.c0 + q);
     ld16(d.y, &s->y.c0 + q);
 }
-__device__ __forceinline__ void ldp(Xyzz<Fp2LaneOps>& d, const Xyzz<Fp2Ops>* s) {
+template <bool C>
+__device__ __forceinline__ void ldp(Xyzz<Fp2LaneOpsT<C>>& d, const Xyzz<Fp2Ops>* s) {
     const uint32_t q = lane_part();
     ld16(d.X, &s->X.c0 + q);
     ld16(d.Y, &s->Y.c0 + q);
@@ -182,7 +186,8 @@ __device__ __forceinline__ void stp(Xyzz<Fp2Ops>* d, const Xyzz<Fp2QuadOps>& v) 
     st16(&d->ZZ.c0 + q, v.ZZ);
     st16(&d->ZZZ.c0 + q, v.ZZZ);
 }
-__device__ __forceinline__ void stp(Xyzz<Fp2Ops>* d, const Xyzz<Fp2LaneOps>& v) {
+template <bool C>
+__device__ __forceinline__ void stp(Xyzz<Fp2Ops>* d, const Xyzz<Fp2LaneOpsT<C>>& v) {
     const uint32_t q = lane_part();
     st16(&d->X.c0 + q, v.X);
     st16(&d->Y.c0 + q, v.Y);
@@ -198,6 +203,17 @@ struct KernelOps {
 template <>
 struct KernelOps<Fp2Ops> {
     using type = Fp2LaneOps;
+};
+// Policy of the accumulation kernel alone: for G2 the lane pairs with
+// single-chain column sums (two waves per SIMD hide the longer dependency
+// chain there; the fix-up and tail kernels keep Fp2LaneOps).
+template <class O>
+struct AccOps {
+    using type = typename KernelOps<O>::type;
+};
+template <>
+struct AccOps<Fp2Ops> {
+    using type = Fp2LaneAccOps;
 };
 
 // ABI affine records (reference include/curve.h:13-25) -> device affine.
