@@ -178,11 +178,19 @@ def bench_host_buffers(bases_dev, scalars_np: np.ndarray, reps: int) -> dict:
             "entry_point": "g1_msm_multi_wrapper (host pointers)"}
 
 
-def bench_served(requests: int) -> dict:
+def bench_served(requests: int, ns_per_point: dict | None = None) -> dict:
     """Continuous-batching server on the mixed stream (Poisson arrivals,
     80% G1 / 20% G2, N log-uniform in [2^10, 2^20], bases resident): a
     moderate rate for latency, a saturating rate for throughput, and the same
-    request list run plain (one MSM at a time)."""
+    request list run plain (one MSM at a time).
+
+    ns_per_point ({curve: ns}, the N = 2^20 MSM's sort and accumulation
+    stages per point, the work that grows with the points and that no
+    batching shares) bounds what any batching of the saturating request list
+    can deliver: every request costs at least its points at that rate, with
+    every per-MSM cost (bucket reduction, window sums, combine, launches) and
+    the larger windows' savings taken as zero. Reported as
+    `ceiling_msms_per_s` and against plain as `ceiling_vs_plain`."""
     from gpu_msm_amd import serve
 
     st = serve.Stream()
@@ -191,7 +199,13 @@ def bench_served(requests: int) -> dict:
         "saturation": serve.run(st, 5000.0, requests, 10, 20, seed=6),
         "plain_same_requests": serve.plain(st, requests, 10, 20, seed=6),
     }
-    out["saturation_vs_plain"] = round(out["saturation"]["msms_per_s"] / out["plain_same_requests"]["msms_per_s"], 2)
+    plain_rate = out["plain_same_requests"]["msms_per_s"]
+    out["saturation_vs_plain"] = round(out["saturation"]["msms_per_s"] / plain_rate, 2)
+    if ns_per_point:
+        _, curves, sizes, _ = serve.make_stream(6, requests, 10, 20, st.pool_n)
+        work_s = sum(int(n) * ns_per_point[int(c)] for c, n in zip(curves, sizes)) * 1e-9
+        out["ceiling_msms_per_s"] = round(requests / work_s, 1)
+        out["ceiling_vs_plain"] = round(requests / work_s / plain_rate, 2)
     st.g1.free()
     st.g2.free()
     return out
@@ -348,7 +362,12 @@ def main() -> None:
     if not args.no_serve:
         print("bench: served mixed stream (80% G1 / 20% G2, N log-uniform in [2^10, 2^20])", file=sys.stderr,
               flush=True)
-        served = bench_served(args.serve_requests)
+        per_point = None
+        if g2 is not None:
+            grow = ("hist", "scan", "scatter", "accumulate")
+            per_point = {gm.CURVE_G1: sum(stage_avg[k] for k in grow) * 1e6 / n,
+                         gm.CURVE_G2: sum(g2["stages_ms"][k] for k in grow) * 1e6 / n}
+        served = bench_served(args.serve_requests, per_point)
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -9,6 +9,9 @@
 #pragma once
 #include <stdint.h>
 #include <string.h>
+#if defined(__x86_64__)
+#include <x86intrin.h>
+#endif
 
 namespace gmh {
 
@@ -50,34 +53,58 @@ inline bool is_zero(const Fp& a) {
 }
 inline bool eq(const Fp& a, const Fp& b) { return memcmp(a.l, b.l, sizeof a.l) == 0; }
 
-inline void add(Fp& r, const Fp& a, const Fp& b) {
-    uint64_t t[7];
-    u128 c = 0;
-    for (int i = 0; i < 7; ++i) {
-        c += (u128)a.l[i] + b.l[i];
-        t[i] = (uint64_t)c;
-        c >>= 64;
-    }
-    if (c || geq_p(t)) sub_p(t);
-    memcpy(r.l, t, sizeof t);
+// Carry and borrow steps of the word chains (adc / sbb on x86-64).
+inline unsigned char addc(unsigned char c, uint64_t a, uint64_t b, uint64_t& o) {
+#if defined(__x86_64__)
+    unsigned long long t;
+    c = _addcarry_u64(c, a, b, &t);
+    o = t;
+    return c;
+#else
+    const u128 s = (u128)a + b + c;
+    o = (uint64_t)s;
+    return (unsigned char)(s >> 64);
+#endif
 }
+inline unsigned char subb(unsigned char br, uint64_t a, uint64_t b, uint64_t& o) {
+#if defined(__x86_64__)
+    unsigned long long t;
+    br = _subborrow_u64(br, a, b, &t);
+    o = t;
+    return br;
+#else
+    const u128 d = (u128)a - b - br;
+    o = (uint64_t)d;
+    return (unsigned char)((d >> 64) & 1);
+#endif
+}
+
+// t - p unless that borrows (t below 2p), masked rather than branched.
+inline void sub_p_once(uint64_t* r, const uint64_t* t) {
+    uint64_t d[7];
+    unsigned char br = 0;
+    for (int i = 0; i < 7; ++i) br = subb(br, t[i], P.l[i], d[i]);
+    const uint64_t keep = 0 - (uint64_t)br;
+    for (int i = 0; i < 7; ++i) r[i] = (t[i] & keep) | (d[i] & ~keep);
+}
+
+// a + b, reduced once: the sum minus p is kept unless that borrows (and no
+// carry left the seventh word). No branch on the values.
+inline void add(Fp& r, const Fp& a, const Fp& b) {
+    uint64_t t[7], d[7];
+    unsigned char c = 0, br = 0;
+    for (int i = 0; i < 7; ++i) c = addc(c, a.l[i], b.l[i], t[i]);
+    for (int i = 0; i < 7; ++i) br = subb(br, t[i], P.l[i], d[i]);
+    const uint64_t keep = 0 - (uint64_t)(br & (c ^ 1));  // all ones: t is already below p
+    for (int i = 0; i < 7; ++i) r.l[i] = (t[i] & keep) | (d[i] & ~keep);
+}
+// a - b, plus p where that borrows (masked, no branch).
 inline void sub(Fp& r, const Fp& a, const Fp& b) {
     uint64_t t[7];
-    uint64_t br = 0;
-    for (int i = 0; i < 7; ++i) {
-        u128 d = (u128)a.l[i] - b.l[i] - br;
-        t[i] = (uint64_t)d;
-        br = (uint64_t)(d >> 64) & 1;
-    }
-    if (br) {
-        u128 c = 0;
-        for (int i = 0; i < 7; ++i) {
-            c += (u128)t[i] + P.l[i];
-            t[i] = (uint64_t)c;
-            c >>= 64;
-        }
-    }
-    memcpy(r.l, t, sizeof t);
+    unsigned char br = 0, c = 0;
+    for (int i = 0; i < 7; ++i) br = subb(br, a.l[i], b.l[i], t[i]);
+    const uint64_t m = 0 - (uint64_t)br;
+    for (int i = 0; i < 7; ++i) c = addc(c, t[i], P.l[i] & m, r.l[i]);
 }
 // -a as p - a, and 0 for 0 (reference src/fp.cu:436-443).
 inline void neg(Fp& r, const Fp& a) {
@@ -471,9 +498,8 @@ inline void mul_adx(Fp& r, const Fp& a, const Fp& b) {
         : "=&r"(x8), "=&r"(x9), "=&r"(x10), "=&r"(x11), "=&r"(x12), "=&r"(x13), "=&r"(x14), "=&r"(x15)
         : [a] "r"(a.l), [b] "r"(b.l), [p] "r"(P.l), [pinv] "m"(PINV)
         : "rax", "rbx", "rdx", "cc", "memory");
-    uint64_t t[7] = {x15, x8, x9, x10, x11, x12, x13};
-    if (geq_p(t)) sub_p(t);
-    memcpy(r.l, t, sizeof r.l);
+    const uint64_t t[7] = {x15, x8, x9, x10, x11, x12, x13};
+    sub_p_once(r.l, t);
 }
 #endif
 
@@ -498,8 +524,7 @@ inline void mul_portable(Fp& r, const Fp& a, const Fp& b) {
         }
         t[6] = t[7] + (uint64_t)c;
     }
-    if (geq_p(t)) sub_p(t);
-    memcpy(r.l, t, sizeof r.l);
+    sub_p_once(r.l, t);
 }
 
 inline bool cpu_has_adx() {
