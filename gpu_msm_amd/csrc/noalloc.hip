@@ -23,9 +23,9 @@
 
 namespace {
 
-// 96 MiB: room for G1 MSMs up to 2^16 points and G2 up to 2^15 with 5-limb
+// 144 MiB: room for G1 MSMs up to 2^16 points and G2 up to 2^15 with 5-limb
 // scalars (the limit is the plan's exact workspace plus the imported bases).
-constexpr size_t SCRATCH_BYTES = (size_t)96 << 20;
+constexpr size_t SCRATCH_BYTES = (size_t)144 << 20;
 static __device__ __align__(256) uint8_t scratch[SCRATCH_BYTES];
 
 constexpr int MAX_DEVICES = 64;

@@ -192,18 +192,23 @@ bool make_plan(Plan& p, MsmDesc* d, int curve, int L, int bits, const MsmSpec* s
     p.win_total = (uint32_t)win;
     p.tile_total = (uint32_t)tile;
     // Entries per accumulation lane, per MSM (chunks never straddle two
-    // MSMs). Parallelism asks for ~64k lanes over the whole batch (chunk 40
-    // for big G1 runs, 80 for G2, whose fix-up adds cost three times more and
-    // whose longer mixed adds hide the gathers with fewer lanes; down to 2 for
-    // small runs, so the serial depth of one mixed add per entry stays
-    // short); the fix-up asks that a typical bucket of the MSM spans at most
-    // ~4 chunks, so chunk >= the MSM's average bucket / 4. At N = 2^20 chunks
-    // of 40 (G1) and 80 (G2) make exactly four rounds of two waves per SIMD
-    // and cut a fifth fewer buckets than 32 / 64: accumulation + fix-up
-    // 4.00-4.03 -> 3.94 ms (G1), 13.58-13.79 -> 13.43 ms (G2); longer chunks
-    // cut fewer buckets still but slow the accumulation more than that.
+    // MSMs). Parallelism asks for at least ~2^17 lanes over the whole batch
+    // (one round of two waves per SIMD for G1, two rounds of lane pairs for
+    // G2): chunk 40 for big G1 runs, 80 for G2, whose fix-up adds cost three
+    // times more and whose longer mixed adds hide the gathers with fewer
+    // lanes, halved down to 2 for smaller runs, so the serial depth of one
+    // mixed add per entry stays short; the fix-up asks that a typical bucket
+    // of the MSM spans at most ~4 chunks, so chunk >= the MSM's average
+    // bucket / 4. At N = 2^20 chunks of 40 (G1) and 80 (G2) make exactly four
+    // rounds of two waves per SIMD and cut a fifth fewer buckets than 32 / 64:
+    // accumulation + fix-up 4.00-4.03 -> 3.94 ms (G1), 13.58-13.79 ->
+    // 13.43 ms (G2); longer chunks cut fewer buckets still but slow the
+    // accumulation more than that (80-160 for G1 and 160 for G2 measured the
+    // same or slower). Below 2^19 points the lane count decides: asking for
+    // 2^16 lanes instead (half as many) measured G2 2^18 6.78-6.83 ms against
+    // 5.66-5.70, G2 2^17 3.93 against 3.46 and G1 2^17 1.68 against 1.56.
     int k_par = curve == CURVE_G2 ? 80 : 40;
-    while (k_par > 2 && rank < (uint64_t)k_par * 65536) k_par >>= 1;
+    while (k_par > 2 && rank < (uint64_t)k_par * 131072) k_par >>= 1;
     uint64_t lanes = 0;
     for (int j = 0; j < count; ++j) {
         MsmDesc& m = d[j];

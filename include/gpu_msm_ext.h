@@ -175,7 +175,7 @@ void gm_server_destroy(gm_server* s);
  * form, d_scalars n x scalar_limbs little-endian u64 words, d_result one
  * canonical affine ABI record, all enqueued on `stream` (a hipStream_t; NULL
  * for the null stream) with no host wait. It calls no hipMalloc / hipFree or
- * allocating library routine: the workspace is one fixed 96 MiB device array
+ * allocating library routine: the workspace is one fixed 144 MiB device array
  * per GPU, held until the process ends, and the window combine runs on the
  * device. Sizes whose workspace does not fit (about 2^16 G1 or 2^15 G2 points
  * with 5-limb scalars) return -6. Calls that overlap on one GPU share that
