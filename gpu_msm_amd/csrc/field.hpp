@@ -18,6 +18,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <utility>
+
 namespace gm {
 
 constexpr int NL = 16;
@@ -248,20 +250,148 @@ constexpr uint32_t HALFL[NL] = {0x5585555, 0x8e00135, 0x9dae618, 0x229462b, 0x6f
 // a^(p-2) (Fermat inverse); 0 maps to 0.
 __device__ inline void fp_inv(Fp& r, const Fp& a) { fp_pow_words(r, a, PM2_WORDS, 14); }
 
-// Column accumulation acc += x * y as one signed (x, y) or unsigned (m, c)
-// mad on the running column sum itself. In plain C++ the compiler sums each
-// column's products on a chain of its own and adds it to the running value
-// afterwards (one more 64-bit add per column); these keep the whole product
-// on one chain. The lane-pair accumulation kernel takes them (CHAIN = true);
-// the depth-bound kernels, whose few waves need the compiler's independent
-// chains to hide the mad latency, keep plain C++. The mad's carry-out is
-// unused (vcc).
-__device__ __forceinline__ void mac_i(int64_t& acc, int32_t x, int32_t y) {
-    asm("v_mad_i64_i32 %0, vcc, %1, %2, %0" : "+v"(acc) : "v"(x), "v"(y) : "vcc");
+// Column accumulation on the running column sum itself. In plain C++ the
+// compiler sums each column's products on a chain of its own and adds it to
+// the running value afterwards (one more 64-bit add per column); these keep
+// the whole product on one chain. The lane-pair accumulation kernel takes
+// them (CHAIN = true); the depth-bound kernels, whose few waves need the
+// compiler's independent chains to hide the mad latency, keep plain C++.
+#if defined(__HIP_DEVICE_COMPILE__)
+// Up to eight mads on the running column sum in one asm statement: the
+// compiler cannot see into an asm statement, so it puts a wait state
+// (s_nop) between consecutive ones; inside one statement consecutive mads
+// need none (a VGPR read after a VALU write interlocks in hardware, and the
+// unused carry-outs only write vcc, which nothing reads). mad_ivN: signed
+// x_j * y_j, both in VGPRs; mad_isN: y_j uniform constants in SGPRs (the
+// reduction products m * p_j have non-negative operands below 2^31, so the
+// signed mad equals the unsigned one). Arguments: x_0..x_{N-1}, y_0..y_{N-1}.
+__device__ __forceinline__ void mad_iv1(int64_t& acc, int32_t x0, int32_t y0) {
+    asm("v_mad_i64_i32 %0, vcc, %1, %2, %0" : "+v"(acc) : "v"(x0), "v"(y0) : "vcc");
 }
-__device__ __forceinline__ void mac_ic(int64_t& acc, uint32_t m, uint32_t c) {
-    asm("v_mad_u64_u32 %0, vcc, %1, %2, %0" : "+v"(acc) : "v"(m), "s"(c) : "vcc");
+__device__ __forceinline__ void mad_iv2(int64_t& acc, int32_t x0, int32_t x1, int32_t y0, int32_t y1) {
+    asm("v_mad_i64_i32 %0, vcc, %1, %3, %0\n\tv_mad_i64_i32 %0, vcc, %2, %4, %0" : "+v"(acc) : "v"(x0), "v"(x1), "v"(y0), "v"(y1) : "vcc");
 }
+__device__ __forceinline__ void mad_iv3(int64_t& acc, int32_t x0, int32_t x1, int32_t x2, int32_t y0, int32_t y1, int32_t y2) {
+    asm("v_mad_i64_i32 %0, vcc, %1, %4, %0\n\tv_mad_i64_i32 %0, vcc, %2, %5, %0\n\tv_mad_i64_i32 %0, vcc, %3, %6, %0" : "+v"(acc) : "v"(x0), "v"(x1), "v"(x2), "v"(y0), "v"(y1), "v"(y2) : "vcc");
+}
+__device__ __forceinline__ void mad_iv4(int64_t& acc, int32_t x0, int32_t x1, int32_t x2, int32_t x3, int32_t y0, int32_t y1, int32_t y2, int32_t y3) {
+    asm("v_mad_i64_i32 %0, vcc, %1, %5, %0\n\tv_mad_i64_i32 %0, vcc, %2, %6, %0\n\tv_mad_i64_i32 %0, vcc, %3, %7, %0\n\tv_mad_i64_i32 %0, vcc, %4, %8, %0" : "+v"(acc) : "v"(x0), "v"(x1), "v"(x2), "v"(x3), "v"(y0), "v"(y1), "v"(y2), "v"(y3) : "vcc");
+}
+__device__ __forceinline__ void mad_iv5(int64_t& acc, int32_t x0, int32_t x1, int32_t x2, int32_t x3, int32_t x4, int32_t y0, int32_t y1, int32_t y2, int32_t y3, int32_t y4) {
+    asm("v_mad_i64_i32 %0, vcc, %1, %6, %0\n\tv_mad_i64_i32 %0, vcc, %2, %7, %0\n\tv_mad_i64_i32 %0, vcc, %3, %8, %0\n\tv_mad_i64_i32 %0, vcc, %4, %9, %0\n\tv_mad_i64_i32 %0, vcc, %5, %10, %0" : "+v"(acc) : "v"(x0), "v"(x1), "v"(x2), "v"(x3), "v"(x4), "v"(y0), "v"(y1), "v"(y2), "v"(y3), "v"(y4) : "vcc");
+}
+__device__ __forceinline__ void mad_iv6(int64_t& acc, int32_t x0, int32_t x1, int32_t x2, int32_t x3, int32_t x4, int32_t x5, int32_t y0, int32_t y1, int32_t y2, int32_t y3, int32_t y4, int32_t y5) {
+    asm("v_mad_i64_i32 %0, vcc, %1, %7, %0\n\tv_mad_i64_i32 %0, vcc, %2, %8, %0\n\tv_mad_i64_i32 %0, vcc, %3, %9, %0\n\tv_mad_i64_i32 %0, vcc, %4, %10, %0\n\tv_mad_i64_i32 %0, vcc, %5, %11, %0\n\tv_mad_i64_i32 %0, vcc, %6, %12, %0" : "+v"(acc) : "v"(x0), "v"(x1), "v"(x2), "v"(x3), "v"(x4), "v"(x5), "v"(y0), "v"(y1), "v"(y2), "v"(y3), "v"(y4), "v"(y5) : "vcc");
+}
+__device__ __forceinline__ void mad_iv7(int64_t& acc, int32_t x0, int32_t x1, int32_t x2, int32_t x3, int32_t x4, int32_t x5, int32_t x6, int32_t y0, int32_t y1, int32_t y2, int32_t y3, int32_t y4, int32_t y5, int32_t y6) {
+    asm("v_mad_i64_i32 %0, vcc, %1, %8, %0\n\tv_mad_i64_i32 %0, vcc, %2, %9, %0\n\tv_mad_i64_i32 %0, vcc, %3, %10, %0\n\tv_mad_i64_i32 %0, vcc, %4, %11, %0\n\tv_mad_i64_i32 %0, vcc, %5, %12, %0\n\tv_mad_i64_i32 %0, vcc, %6, %13, %0\n\tv_mad_i64_i32 %0, vcc, %7, %14, %0" : "+v"(acc) : "v"(x0), "v"(x1), "v"(x2), "v"(x3), "v"(x4), "v"(x5), "v"(x6), "v"(y0), "v"(y1), "v"(y2), "v"(y3), "v"(y4), "v"(y5), "v"(y6) : "vcc");
+}
+__device__ __forceinline__ void mad_iv8(int64_t& acc, int32_t x0, int32_t x1, int32_t x2, int32_t x3, int32_t x4, int32_t x5, int32_t x6, int32_t x7, int32_t y0, int32_t y1, int32_t y2, int32_t y3, int32_t y4, int32_t y5, int32_t y6, int32_t y7) {
+    asm("v_mad_i64_i32 %0, vcc, %1, %9, %0\n\tv_mad_i64_i32 %0, vcc, %2, %10, %0\n\tv_mad_i64_i32 %0, vcc, %3, %11, %0\n\tv_mad_i64_i32 %0, vcc, %4, %12, %0\n\tv_mad_i64_i32 %0, vcc, %5, %13, %0\n\tv_mad_i64_i32 %0, vcc, %6, %14, %0\n\tv_mad_i64_i32 %0, vcc, %7, %15, %0\n\tv_mad_i64_i32 %0, vcc, %8, %16, %0" : "+v"(acc) : "v"(x0), "v"(x1), "v"(x2), "v"(x3), "v"(x4), "v"(x5), "v"(x6), "v"(x7), "v"(y0), "v"(y1), "v"(y2), "v"(y3), "v"(y4), "v"(y5), "v"(y6), "v"(y7) : "vcc");
+}
+__device__ __forceinline__ void mad_is1(int64_t& acc, int32_t x0, int32_t y0) {
+    asm("v_mad_i64_i32 %0, vcc, %1, %2, %0" : "+v"(acc) : "v"(x0), "s"(y0) : "vcc");
+}
+__device__ __forceinline__ void mad_is2(int64_t& acc, int32_t x0, int32_t x1, int32_t y0, int32_t y1) {
+    asm("v_mad_i64_i32 %0, vcc, %1, %3, %0\n\tv_mad_i64_i32 %0, vcc, %2, %4, %0" : "+v"(acc) : "v"(x0), "v"(x1), "s"(y0), "s"(y1) : "vcc");
+}
+__device__ __forceinline__ void mad_is3(int64_t& acc, int32_t x0, int32_t x1, int32_t x2, int32_t y0, int32_t y1, int32_t y2) {
+    asm("v_mad_i64_i32 %0, vcc, %1, %4, %0\n\tv_mad_i64_i32 %0, vcc, %2, %5, %0\n\tv_mad_i64_i32 %0, vcc, %3, %6, %0" : "+v"(acc) : "v"(x0), "v"(x1), "v"(x2), "s"(y0), "s"(y1), "s"(y2) : "vcc");
+}
+__device__ __forceinline__ void mad_is4(int64_t& acc, int32_t x0, int32_t x1, int32_t x2, int32_t x3, int32_t y0, int32_t y1, int32_t y2, int32_t y3) {
+    asm("v_mad_i64_i32 %0, vcc, %1, %5, %0\n\tv_mad_i64_i32 %0, vcc, %2, %6, %0\n\tv_mad_i64_i32 %0, vcc, %3, %7, %0\n\tv_mad_i64_i32 %0, vcc, %4, %8, %0" : "+v"(acc) : "v"(x0), "v"(x1), "v"(x2), "v"(x3), "s"(y0), "s"(y1), "s"(y2), "s"(y3) : "vcc");
+}
+__device__ __forceinline__ void mad_is5(int64_t& acc, int32_t x0, int32_t x1, int32_t x2, int32_t x3, int32_t x4, int32_t y0, int32_t y1, int32_t y2, int32_t y3, int32_t y4) {
+    asm("v_mad_i64_i32 %0, vcc, %1, %6, %0\n\tv_mad_i64_i32 %0, vcc, %2, %7, %0\n\tv_mad_i64_i32 %0, vcc, %3, %8, %0\n\tv_mad_i64_i32 %0, vcc, %4, %9, %0\n\tv_mad_i64_i32 %0, vcc, %5, %10, %0" : "+v"(acc) : "v"(x0), "v"(x1), "v"(x2), "v"(x3), "v"(x4), "s"(y0), "s"(y1), "s"(y2), "s"(y3), "s"(y4) : "vcc");
+}
+__device__ __forceinline__ void mad_is6(int64_t& acc, int32_t x0, int32_t x1, int32_t x2, int32_t x3, int32_t x4, int32_t x5, int32_t y0, int32_t y1, int32_t y2, int32_t y3, int32_t y4, int32_t y5) {
+    asm("v_mad_i64_i32 %0, vcc, %1, %7, %0\n\tv_mad_i64_i32 %0, vcc, %2, %8, %0\n\tv_mad_i64_i32 %0, vcc, %3, %9, %0\n\tv_mad_i64_i32 %0, vcc, %4, %10, %0\n\tv_mad_i64_i32 %0, vcc, %5, %11, %0\n\tv_mad_i64_i32 %0, vcc, %6, %12, %0" : "+v"(acc) : "v"(x0), "v"(x1), "v"(x2), "v"(x3), "v"(x4), "v"(x5), "s"(y0), "s"(y1), "s"(y2), "s"(y3), "s"(y4), "s"(y5) : "vcc");
+}
+__device__ __forceinline__ void mad_is7(int64_t& acc, int32_t x0, int32_t x1, int32_t x2, int32_t x3, int32_t x4, int32_t x5, int32_t x6, int32_t y0, int32_t y1, int32_t y2, int32_t y3, int32_t y4, int32_t y5, int32_t y6) {
+    asm("v_mad_i64_i32 %0, vcc, %1, %8, %0\n\tv_mad_i64_i32 %0, vcc, %2, %9, %0\n\tv_mad_i64_i32 %0, vcc, %3, %10, %0\n\tv_mad_i64_i32 %0, vcc, %4, %11, %0\n\tv_mad_i64_i32 %0, vcc, %5, %12, %0\n\tv_mad_i64_i32 %0, vcc, %6, %13, %0\n\tv_mad_i64_i32 %0, vcc, %7, %14, %0" : "+v"(acc) : "v"(x0), "v"(x1), "v"(x2), "v"(x3), "v"(x4), "v"(x5), "v"(x6), "s"(y0), "s"(y1), "s"(y2), "s"(y3), "s"(y4), "s"(y5), "s"(y6) : "vcc");
+}
+__device__ __forceinline__ void mad_is8(int64_t& acc, int32_t x0, int32_t x1, int32_t x2, int32_t x3, int32_t x4, int32_t x5, int32_t x6, int32_t x7, int32_t y0, int32_t y1, int32_t y2, int32_t y3, int32_t y4, int32_t y5, int32_t y6, int32_t y7) {
+    asm("v_mad_i64_i32 %0, vcc, %1, %9, %0\n\tv_mad_i64_i32 %0, vcc, %2, %10, %0\n\tv_mad_i64_i32 %0, vcc, %3, %11, %0\n\tv_mad_i64_i32 %0, vcc, %4, %12, %0\n\tv_mad_i64_i32 %0, vcc, %5, %13, %0\n\tv_mad_i64_i32 %0, vcc, %6, %14, %0\n\tv_mad_i64_i32 %0, vcc, %7, %15, %0\n\tv_mad_i64_i32 %0, vcc, %8, %16, %0" : "+v"(acc) : "v"(x0), "v"(x1), "v"(x2), "v"(x3), "v"(x4), "v"(x5), "v"(x6), "v"(x7), "s"(y0), "s"(y1), "s"(y2), "s"(y3), "s"(y4), "s"(y5), "s"(y6), "s"(y7) : "vcc");
+}
+
+template <bool SGPR_Y, class... A>
+__device__ __forceinline__ void mad_blk(int64_t& acc, A... a) {
+    constexpr int N = (int)sizeof...(A) / 2;
+    static_assert(N >= 1 && N <= 8, "one to eight mads per statement");
+    if constexpr (SGPR_Y) {
+        if constexpr (N == 1) mad_is1(acc, a...);
+        else if constexpr (N == 2) mad_is2(acc, a...);
+        else if constexpr (N == 3) mad_is3(acc, a...);
+        else if constexpr (N == 4) mad_is4(acc, a...);
+        else if constexpr (N == 5) mad_is5(acc, a...);
+        else if constexpr (N == 6) mad_is6(acc, a...);
+        else if constexpr (N == 7) mad_is7(acc, a...);
+        else mad_is8(acc, a...);
+    } else {
+        if constexpr (N == 1) mad_iv1(acc, a...);
+        else if constexpr (N == 2) mad_iv2(acc, a...);
+        else if constexpr (N == 3) mad_iv3(acc, a...);
+        else if constexpr (N == 4) mad_iv4(acc, a...);
+        else if constexpr (N == 5) mad_iv5(acc, a...);
+        else if constexpr (N == 6) mad_iv6(acc, a...);
+        else if constexpr (N == 7) mad_iv7(acc, a...);
+        else mad_iv8(acc, a...);
+    }
+}
+
+// Column K of a product scan over T operand pairs (x_u, y_u): the limb
+// products x_u[i] * y_u[K - i] for i from col_ilo(K) to col_ihi(K), numbered
+// j = (i - col_ilo(K)) * T + u, issued in statements of up to eight.
+constexpr int col_ilo(int K) { return K < NL ? 0 : K - NL + 1; }
+constexpr int col_ihi(int K) { return K < NL ? K : NL - 1; }
+
+template <int T, int K, int J0, size_t... Is>
+__device__ __forceinline__ void col_prod_blk(int64_t& acc, const Fp* const (&x)[T],
+                                             const int32_t (*const (&y)[T])[NL], std::index_sequence<Is...>) {
+    mad_blk<false>(acc, (int32_t)x[(J0 + (int)Is) % T]->v[col_ilo(K) + (J0 + (int)Is) / T]...,
+                   (*y[(J0 + (int)Is) % T])[K - col_ilo(K) - (J0 + (int)Is) / T]...);
+}
+template <int T, int K, int J0 = 0>
+__device__ __forceinline__ void col_prods(int64_t& acc, const Fp* const (&x)[T], const int32_t (*const (&y)[T])[NL]) {
+    constexpr int NT = (col_ihi(K) - col_ilo(K) + 1) * T;
+    if constexpr (J0 < NT) {
+        constexpr int N = NT - J0 < 8 ? NT - J0 : 8;
+        col_prod_blk<T, K, J0>(acc, x, y, std::make_index_sequence<N>{});
+        col_prods<T, K, J0 + N>(acc, x, y);
+    }
+}
+// Reduction products m[i] * p[K - i] of column K for i in [I0, I1).
+template <int K, int I0, size_t... Is>
+__device__ __forceinline__ void col_red_blk(int64_t& acc, const uint32_t (&m)[NL], std::index_sequence<Is...>) {
+    mad_blk<true>(acc, (int32_t)m[I0 + (int)Is]..., (int32_t)PL[K - I0 - (int)Is]...);
+}
+template <int K, int I0, int I1>
+__device__ __forceinline__ void col_reds(int64_t& acc, const uint32_t (&m)[NL]) {
+    if constexpr (I0 < I1) {
+        constexpr int N = I1 - I0 < 8 ? I1 - I0 : 8;
+        col_red_blk<K, I0>(acc, m, std::make_index_sequence<N>{});
+        col_reds<K, I0 + N, I1>(acc, m);
+    }
+}
+// The single-chain Montgomery product scan, one column per step.
+template <int T, int K>
+__device__ __forceinline__ void redc_chain_cols(uint32_t (&t)[NL], int64_t& acc, uint32_t (&m)[NL],
+                                                const Fp* const (&x)[T], const int32_t (*const (&y)[T])[NL]) {
+    if constexpr (K < 2 * NL - 1) {
+        col_prods<T, K>(acc, x, y);
+        if constexpr (K < NL) {
+            col_reds<K, 0, K>(acc, m);
+            m[K] = ((uint32_t)acc * PINV28) & MASK28;
+            mad_blk<true>(acc, (int32_t)m[K], (int32_t)PL[0]);
+        } else {
+            col_reds<K, K - NL + 1, NL>(acc, m);
+            t[K - NL] = (uint32_t)acc & MASK28;
+        }
+        acc >>= 28;  // arithmetic: for K < NL the low 28 bits are zero
+        redc_chain_cols<T, K + 1>(t, acc, m, x, y);
+    }
+}
+#endif
 
 // Signed Montgomery reduction of a sum of limb products, the core of the
 // lazily reduced products below: value = sum over the T terms of x_t * y_t,
@@ -276,34 +406,24 @@ __host__ __device__ __forceinline__ void redc_signed(uint32_t (&t)[NL], int64_t&
                                             const int32_t (*const (&y)[T])[NL]) {
     uint32_t m[NL];
     int64_t acc = 0;
-    auto mx = [&](int32_t a, int32_t b) {
 #if defined(__HIP_DEVICE_COMPILE__)
-        if constexpr (CHAIN) {
-            mac_i(acc, a, b);
-            return;
-        }
+    if constexpr (CHAIN) {
+        // each column's limb products first, then its reduction products
+        redc_chain_cols<T, 0>(t, acc, m, x, y);
+        top = acc;
+        return;
+    }
 #endif
-        acc += (int64_t)a * b;
-    };
-    auto mp = [&](uint32_t a, uint32_t c) {
-#if defined(__HIP_DEVICE_COMPILE__)
-        if constexpr (CHAIN) {
-            mac_ic(acc, a, c);
-            return;
-        }
-#endif
-        acc += (int64_t)((uint64_t)a * c);
-    };
 #pragma unroll
     for (int k = 0; k < NL; ++k) {
 #pragma unroll
         for (int i = 0; i <= k; ++i)
 #pragma unroll
-            for (int u = 0; u < T; ++u) mx((int32_t)x[u]->v[i], (*y[u])[k - i]);
+            for (int u = 0; u < T; ++u) acc += (int64_t)(int32_t)x[u]->v[i] * (*y[u])[k - i];
 #pragma unroll
-        for (int i = 0; i < k; ++i) mp(m[i], PL[k - i]);
+        for (int i = 0; i < k; ++i) acc += (int64_t)((uint64_t)m[i] * PL[k - i]);
         m[k] = ((uint32_t)acc * PINV28) & MASK28;
-        mp(m[k], PL[0]);
+        acc += (int64_t)((uint64_t)m[k] * PL[0]);
         acc >>= 28;  // arithmetic: the low 28 bits are zero
     }
 #pragma unroll
@@ -311,8 +431,8 @@ __host__ __device__ __forceinline__ void redc_signed(uint32_t (&t)[NL], int64_t&
 #pragma unroll
         for (int i = k - NL + 1; i < NL; ++i) {
 #pragma unroll
-            for (int u = 0; u < T; ++u) mx((int32_t)x[u]->v[i], (*y[u])[k - i]);
-            mp(m[i], PL[k - i]);
+            for (int u = 0; u < T; ++u) acc += (int64_t)(int32_t)x[u]->v[i] * (*y[u])[k - i];
+            acc += (int64_t)((uint64_t)m[i] * PL[k - i]);
         }
         t[k - NL] = (uint32_t)acc & MASK28;
         acc >>= 28;
