@@ -38,6 +38,10 @@ __device__ __forceinline__ bool aff_is_inf<FpOps>(const Aff<FpOps>& p) {
     return p.x.v[NL - 1] == INF_MARK;
 }
 template <>
+__device__ __forceinline__ bool aff_is_inf<FpAccOps>(const Aff<FpAccOps>& p) {
+    return p.x.v[NL - 1] == INF_MARK;
+}
+template <>
 __device__ __forceinline__ bool aff_is_inf<Fp2Ops>(const Aff<Fp2Ops>& p) {
     return p.x.c0.v[NL - 1] == INF_MARK;
 }

@@ -253,9 +253,10 @@ __device__ inline void fp_inv(Fp& r, const Fp& a) { fp_pow_words(r, a, PM2_WORDS
 // Column accumulation on the running column sum itself. In plain C++ the
 // compiler sums each column's products on a chain of its own and adds it to
 // the running value afterwards (one more 64-bit add per column); these keep
-// the whole product on one chain. The lane-pair accumulation kernel takes
-// them (CHAIN = true); the depth-bound kernels, whose few waves need the
-// compiler's independent chains to hide the mad latency, keep plain C++.
+// the whole product on one chain. Both accumulation kernels take them (the
+// G2 lane pairs through CHAIN = true, G1 through FpAccOps); the depth-bound
+// kernels, whose few waves need the compiler's independent chains to hide
+// the mad latency, keep plain C++.
 #if defined(__HIP_DEVICE_COMPILE__)
 // Up to eight mads on the running column sum in one asm statement: the
 // compiler cannot see into an asm statement, so it puts a wait state
@@ -389,6 +390,62 @@ __device__ __forceinline__ void redc_chain_cols(uint32_t (&t)[NL], int64_t& acc,
         }
         acc >>= 28;  // arithmetic: for K < NL the low 28 bits are zero
         redc_chain_cols<T, K + 1>(t, acc, m, x, y);
+    }
+}
+
+// Column K of a square of s (d = 2s): the off-diagonal products d[i] s[K - i]
+// for i < K - i and, for even K, the diagonal s[K/2]^2, numbered j from 0
+// (the diagonal last), issued in statements of up to eight.
+template <int K>
+struct SqCol {
+    static constexpr int ilo = col_ilo(K);
+    static constexpr int imax = (K + 1) / 2 - 1;  // largest i with i < K - i
+    static constexpr int npair = imax >= ilo ? imax - ilo + 1 : 0;
+    static constexpr int n = npair + (K % 2 == 0 ? 1 : 0);
+};
+template <int K, int J>
+__device__ __forceinline__ int32_t sq_x(const int32_t (&d)[NL], const int32_t (&s)[NL]) {
+    if constexpr (J < SqCol<K>::npair) return d[SqCol<K>::ilo + J];
+    else return s[K / 2];
+}
+template <int K, int J>
+__device__ __forceinline__ int32_t sq_y(const int32_t (&s)[NL]) {
+    if constexpr (J < SqCol<K>::npair) return s[K - SqCol<K>::ilo - J];
+    else return s[K / 2];
+}
+template <int K, int J0, size_t... Is>
+__device__ __forceinline__ void sq_col_blk(int64_t& acc, const int32_t (&d)[NL], const int32_t (&s)[NL],
+                                           std::index_sequence<Is...>) {
+    mad_blk<false>(acc, sq_x<K, J0 + (int)Is>(d, s)..., sq_y<K, J0 + (int)Is>(s)...);
+}
+template <int K, int J0 = 0>
+__device__ __forceinline__ void sq_col_prods(int64_t& acc, const int32_t (&d)[NL], const int32_t (&s)[NL]) {
+    constexpr int NT = SqCol<K>::n;
+    if constexpr (J0 < NT) {
+        constexpr int N = NT - J0 < 8 ? NT - J0 : 8;
+        sq_col_blk<K, J0>(acc, d, s, std::make_index_sequence<N>{});
+        sq_col_prods<K, J0 + N>(acc, d, s);
+    }
+}
+// The single-chain Montgomery square scan. With SUB, h[K - NL] joins column
+// K of the high half (the fused X3 below); h[NL - 1] is left to the caller.
+template <int K, bool SUB>
+__device__ __forceinline__ void redc_sqr_chain_cols(uint32_t (&t)[NL], int64_t& acc, uint32_t (&m)[NL],
+                                                    const int32_t (&d)[NL], const int32_t (&s)[NL],
+                                                    const int32_t (&h)[NL]) {
+    if constexpr (K < 2 * NL - 1) {
+        sq_col_prods<K>(acc, d, s);
+        if constexpr (K < NL) {
+            col_reds<K, 0, K>(acc, m);
+            m[K] = ((uint32_t)acc * PINV28) & MASK28;
+            mad_blk<true>(acc, (int32_t)m[K], (int32_t)PL[0]);
+        } else {
+            col_reds<K, K - NL + 1, NL>(acc, m);
+            if constexpr (SUB) acc += (int64_t)h[K - NL];
+            t[K - NL] = (uint32_t)acc & MASK28;
+        }
+        acc >>= 28;
+        redc_sqr_chain_cols<K + 1, SUB>(t, acc, m, d, s, h);
     }
 }
 #endif
@@ -824,6 +881,114 @@ struct FpOps {
     __device__ static __forceinline__ void inv(F& r, const F& a) { fp_inv(r, a); }
     __device__ static __forceinline__ void zero(F& r) { fp_set_zero(r); }
     __device__ static __forceinline__ void one(F& r) { fp_set_one(r); }
+};
+
+// Single-chain forms of the G1 products of the mixed addition: the same
+// values and bounds as fp_mul, fp_mul_raw, fp_mul_sub_raw, fp_sqr_raw and
+// fp_sqr_raw_sub2, every column summed on the running column sum itself.
+// The host pass (which never runs them) compiles the plain forms.
+__device__ __forceinline__ void fp_mul_chain(Fp& r, const Fp& a, const Fp& b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const Fp* const x[1] = {&a};
+    const int32_t(*const y[1])[NL] = {reinterpret_cast<const int32_t(*)[NL]>(&b.v)};
+    uint32_t t[NL];
+    int64_t top;
+    redc_signed<1, true>(t, top, x, y);
+#pragma unroll
+    for (int i = 0; i < NL - 1; ++i) r.v[i] = t[i];
+    r.v[NL - 1] = (uint32_t)top;  // a b < 4p^2 reduces to [0, 2p)
+#else
+    fp_mul(r, a, b);
+#endif
+}
+__device__ __forceinline__ void fp_mul_raw_chain(Fp& r, const Fp& a, const Fp& b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const Fp* const x[1] = {&a};
+    const int32_t(*const y[1])[NL] = {reinterpret_cast<const int32_t(*)[NL]>(&b.v)};
+    uint32_t t[NL];
+    int64_t top;
+    redc_signed<1, true>(t, top, x, y);
+    redc_fold(r, t, top, false);
+#else
+    fp_mul_raw(r, a, b);
+#endif
+}
+__device__ __forceinline__ void fp_mul_sub_raw_chain(Fp& r, const Fp& a, const Fp& b, const Fp& c, const Fp& d) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    int32_t nd[NL];
+#pragma unroll
+    for (int i = 0; i < NL; ++i) nd[i] = -(int32_t)d.v[i];
+    const Fp* const x[2] = {&a, &c};
+    const int32_t(*const y[2])[NL] = {reinterpret_cast<const int32_t(*)[NL]>(&b.v), &nd};
+    uint32_t t[NL];
+    int64_t top;
+    redc_signed<2, true>(t, top, x, y);
+    redc_fold(r, t, top, false);
+#else
+    fp_mul_sub_raw(r, a, b, c, d);
+#endif
+}
+__device__ __forceinline__ void fp_sqr_raw_chain(Fp& r, const Fp& a) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    uint32_t m[NL], t[NL];
+    int32_t d[NL], s[NL];
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+        s[i] = (int32_t)a.v[i];
+        d[i] = s[i] * 2;
+    }
+    int64_t acc = 0;
+    redc_sqr_chain_cols<0, false>(t, acc, m, d, s, s);
+#pragma unroll
+    for (int i = 0; i < NL - 1; ++i) r.v[i] = t[i];
+    r.v[NL - 1] = (uint32_t)acc;  // the square is in [0, 2p): the top limb is non-negative
+#else
+    fp_sqr_raw(r, a);
+#endif
+}
+// a^2 - b - 2c for a raw a (fp_sqr_raw_sub2: (0, 8p) before the two
+// conditional subtractions).
+__device__ __forceinline__ void fp_sqr_raw_sub2_chain(Fp& r, const Fp& a, const Fp& b, const Fp& c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    uint32_t m[NL], t[NL];
+    int32_t d[NL], s[NL], h[NL];
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+        s[i] = (int32_t)a.v[i];
+        d[i] = s[i] * 2;
+        h[i] = (int32_t)(6u * PL[i]) - (int32_t)b.v[i] - 2 * (int32_t)c.v[i];
+    }
+    int64_t acc = 0;
+    redc_sqr_chain_cols<0, true>(t, acc, m, d, s, h);
+    acc += (int64_t)h[NL - 1];
+    t[NL - 1] = (uint32_t)acc;  // in (0, 8p): non-negative, below 2^29
+    fp_cond_sub(t, P4L);
+    fp_cond_sub(t, P2L);
+#pragma unroll
+    for (int i = 0; i < NL; ++i) r.v[i] = t[i];
+#else
+    fp_sqr_raw_sub2(r, a, b, c);
+#endif
+}
+
+// G1 policy of the accumulation kernel: FpOps with the single-chain products
+// (the kernel runs at two waves per SIMD, whose second wave hides the longer
+// dependency chain).
+struct FpAccOps : FpOps {
+    __device__ static __forceinline__ void mul(F& r, const F& a, const F& b) { fp_mul_chain(r, a, b); }
+    __device__ static __forceinline__ void sqr_raw(F& r, const F& a) { fp_sqr_raw_chain(r, a); }
+    __device__ static __forceinline__ void mul_raw(F& r, const F& a, const F& b) { fp_mul_raw_chain(r, a, b); }
+    __device__ static __forceinline__ void mul_sub_raw(F& r, const F& a, const F& b, const F& c, const F& d) {
+        fp_mul_sub_raw_chain(r, a, b, c, d);
+    }
+    __device__ static __forceinline__ void sqr_sub2(F& r, const F& a, const F& b, const F& c) {
+        fp_sqr_raw_sub2_chain(r, a, b, c);
+    }
+    __device__ static __forceinline__ void sqr_sub2z(F& r, const F& a, const F& c) {
+        Fp z;
+        fp_set_zero(z);
+        fp_sqr_raw_sub2_chain(r, a, z, c);
+    }
 };
 
 struct Fp2Ops {

@@ -122,6 +122,13 @@ struct Stor<Fp2LaneOps> {
 };
 template <>
 struct Stor<Fp2LaneAccOps> : Stor<Fp2LaneOps> {};
+// The G1 accumulation's single-chain policy stores plain G1 records.
+template <>
+struct Stor<FpAccOps> {
+    using A = Aff<FpOps>;
+    using X = Xyzz<FpOps>;
+    static constexpr int LPI = 1;
+};
 // Tail kernels on split lane groups: G1 two lanes per item, each holding the
 // whole record; G2 four (two lane pairs), each pair holding the record.
 template <>
@@ -162,6 +169,12 @@ __device__ __forceinline__ void ldp(Xyzz<Fp2LaneOpsT<C>>& d, const Xyzz<Fp2Ops>*
     ld16(d.Y, &s->Y.c0 + q);
     ld16(d.ZZ, &s->ZZ.c0 + q);
     ld16(d.ZZZ, &s->ZZZ.c0 + q);
+}
+__device__ __forceinline__ void ldp(Aff<FpAccOps>& d, const Aff<FpOps>* s) {
+    ld16(d, reinterpret_cast<const Aff<FpAccOps>*>(s));
+}
+__device__ __forceinline__ void stp(Xyzz<FpOps>* d, const Xyzz<FpAccOps>& v) {
+    st16(reinterpret_cast<Xyzz<FpAccOps>*>(d), v);
 }
 __device__ __forceinline__ void ldp(Xyzz<FpPairOps>& d, const Xyzz<FpOps>* s) {
     ld16(d, reinterpret_cast<const Xyzz<FpPairOps>*>(s));
@@ -214,6 +227,10 @@ struct AccOps {
 template <>
 struct AccOps<Fp2Ops> {
     using type = Fp2LaneAccOps;
+};
+template <>
+struct AccOps<FpOps> {
+    using type = FpAccOps;
 };
 
 // ABI affine records (reference include/curve.h:13-25) -> device affine.
