@@ -20,6 +20,7 @@
 #include "host_combine.hpp"
 #include "host_field.hpp"
 #include "resident.hpp"
+#include "staging.hpp"
 
 #define GM_TRY_HOST(x)                   \
     do {                                 \
@@ -46,19 +47,6 @@ using gmc::host_combine;
 
 namespace {
 
-// Pinned staging for host -> device copies of caller (pageable) buffers: one
-// pair of slots per worker thread, each slot with the event of its last DMA.
-struct Staging {
-    static constexpr int WORKERS = 8;
-    static constexpr int SLOTS = 2;
-    static constexpr size_t SLOT_BYTES = (size_t)4 << 20;
-    // copies below this size go straight to hipMemcpyAsync
-    static constexpr size_t MIN_BYTES = (size_t)8 << 20;
-    uint8_t* buf = nullptr;  // WORKERS * SLOTS * SLOT_BYTES
-    hipEvent_t ev[WORKERS][SLOTS] = {};
-    bool failed = false;  // allocation failed once: plain copies from then on
-};
-
 struct DevCtx {
     int dev = 0;
     hipStream_t stream = nullptr;
@@ -66,7 +54,7 @@ struct DevCtx {
     size_t cap = 0;
     uint8_t* pinned = nullptr;  // host staging for window sums / results
     size_t pinned_cap = 0;
-    Staging stage;
+    gmc::Staging stage;
     std::mutex mu;
     gm::Prof prof{};
     bool prof_valid = false;
@@ -136,102 +124,14 @@ hipError_t ensure_pinned(DevCtx* c, size_t bytes) {
     return hipSuccess;
 }
 
-bool ensure_staging(DevCtx* c) {
-    Staging& st = c->stage;
-    if (st.buf) return true;
-    if (st.failed) return false;
-    const size_t bytes = (size_t)Staging::WORKERS * Staging::SLOTS * Staging::SLOT_BYTES;
-    if (hipHostMalloc((void**)&st.buf, bytes, hipHostMallocDefault) != hipSuccess) {
-        st.buf = nullptr;
-        st.failed = true;
-        return false;
-    }
-    for (int w = 0; w < Staging::WORKERS; ++w)
-        for (int k = 0; k < Staging::SLOTS; ++k)
-            if (hipEventCreateWithFlags(&st.ev[w][k], hipEventDisableTiming) != hipSuccess) {
-                st.ev[w][k] = nullptr;
-                for (auto& row : st.ev)
-                    for (hipEvent_t& e : row)
-                        if (e) {
-                            hipEventDestroy(e);
-                            e = nullptr;
-                        }
-                hipHostFree(st.buf);
-                st.buf = nullptr;
-                st.failed = true;
-                return false;
-            }
-    return true;
-}
-
-// One host -> device copy of a pageable caller buffer.
-struct H2D {
-    void* dst;
-    const void* src;
-    size_t bytes;
-};
-
-// Host -> device copies of pageable caller buffers on c->stream. Large copies
-// are split into one contiguous part of their concatenation per worker
-// thread; each worker copies its part slot by slot into pinned memory
-// (several host threads together move pageable memory faster than one) and
-// queues each slot's DMA on the stream, waiting on the slot's previous DMA
-// before refilling it, so host copies and DMAs overlap. Every DMA is queued
-// before this returns, so work queued on the stream afterwards sees the data;
-// the caller's buffers are no longer read. Caller holds c->mu.
-hipError_t h2d_staged(DevCtx* c, const H2D* cp, int ncp) {
-    size_t total = 0;
-    for (int i = 0; i < ncp; ++i) total += cp[i].bytes;
-    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-    const int workers = (int)std::min<size_t>(std::min<unsigned>(Staging::WORKERS, hw), total / Staging::SLOT_BYTES);
-    if (total < Staging::MIN_BYTES || workers < 2 || !ensure_staging(c)) {
-        for (int i = 0; i < ncp; ++i)
-            if (cp[i].bytes) {
-                const hipError_t e = hipMemcpyAsync(cp[i].dst, cp[i].src, cp[i].bytes, hipMemcpyHostToDevice, c->stream);
-                if (e != hipSuccess) return e;
-            }
-        return hipSuccess;
-    }
-    Staging& st = c->stage;
-    const size_t part = (total + workers - 1) / workers;
-    std::vector<hipError_t> err(workers, hipSuccess);
-    auto run = [&](int w) {
-        hipError_t e = hipSetDevice(c->dev);
-        const size_t a = std::min(total, (size_t)w * part), b = std::min(total, a + part);
-        int k = 0;
-        size_t base = 0;  // offset of copy i in the concatenation
-        for (int i = 0; i < ncp && e == hipSuccess; base += cp[i].bytes, ++i) {
-            const size_t lo = std::max(a, base), hi = std::min(b, base + cp[i].bytes);
-            for (size_t o = lo; o < hi && e == hipSuccess; o += Staging::SLOT_BYTES, k ^= 1) {
-                const size_t len = std::min(Staging::SLOT_BYTES, hi - o);
-                uint8_t* slot = st.buf + ((size_t)w * Staging::SLOTS + k) * Staging::SLOT_BYTES;
-                e = hipEventSynchronize(st.ev[w][k]);  // the slot's previous DMA is done
-                if (e != hipSuccess) break;
-                memcpy(slot, (const uint8_t*)cp[i].src + (o - base), len);
-                e = hipMemcpyAsync((uint8_t*)cp[i].dst + (o - base), slot, len, hipMemcpyHostToDevice, c->stream);
-                if (e == hipSuccess) e = hipEventRecord(st.ev[w][k], c->stream);
-            }
-        }
-        err[w] = e;
-    };
-    std::vector<std::thread> th;
-    int started = 1;
-    try {
-        th.reserve(workers - 1);
-        for (; started < workers; ++started) th.emplace_back(run, started);
-    } catch (...) {
-        // a thread that cannot start leaves its part (and the later ones) to this thread
-    }
-    for (int w = started; w < workers; ++w) run(w);
-    run(0);
-    for (auto& t : th) t.join();
-    for (hipError_t e : err)
-        if (e != hipSuccess) return e;
-    return hipSuccess;
+// Host -> device copies of pageable caller buffers on c->stream through the
+// context's pinned staging. Caller holds c->mu.
+hipError_t h2d_staged(DevCtx* c, const gmc::H2D* cp, int ncp) {
+    return gmc::h2d_staged(c->stage, c->dev, c->stream, cp, ncp);
 }
 
 hipError_t h2d_staged(DevCtx* c, void* dst, const void* src, size_t bytes) {
-    const H2D one = {dst, src, bytes};
+    const gmc::H2D one = {dst, src, bytes};
     return h2d_staged(c, &one, 1);
 }
 
@@ -321,7 +221,7 @@ int msm_host_ptrs(void* result_proj, const void* points, const uint64_t* scalars
     Aff<O>* d_aff = reinterpret_cast<Aff<O>*>(c->ws + b_abi);
     uint64_t* d_sc = reinterpret_cast<uint64_t*>(c->ws + b_abi + b_aff);
     uint8_t* ws = c->ws + b_abi + b_aff + b_sc;
-    const H2D cp[2] = {{d_abi, points, (size_t)n * Lay::AFF_BYTES}, {d_sc, scalars, (size_t)n * L * 8}};
+    const gmc::H2D cp[2] = {{d_abi, points, (size_t)n * Lay::AFF_BYTES}, {d_sc, scalars, (size_t)n * L * 8}};
     if (h2d_staged(c, cp, 2) != hipSuccess) return -4;
     if (gm::Engine<O>::import_aff(c->stream, d_aff, d_abi, n, to_mont) != hipSuccess) return -4;
     std::vector<AffResult<HF>> res;

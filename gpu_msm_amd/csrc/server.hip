@@ -14,10 +14,14 @@
 // behind large MSMs (a 2^20 G2 MSM holds its slot for some 16 ms). Every
 // batch holds one curve, scalar width and size class.
 //
-// One completion thread per slot waits for that slot's batch, runs its window
-// combines on host threads and publishes each request's affine result and
-// latency (submit -> result ready), so batches complete in whatever order
-// they finish, not in launch order.
+// The scheduler only forms batches. A worker thread per slot stages the
+// batch's scalars through the slot's pinned buffers (several host threads;
+// the caller's memory is pageable), queues the pipeline on the slot's stream,
+// waits for it and releases the slot as soon as the device is done; the window
+// combines then run on a pool of host threads, which publish each request's
+// affine result and latency (submit -> result ready). Batches complete in
+// whatever order they finish, not in launch order, and a slot takes its next
+// batch while the host still combines the last one.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <string.h>
@@ -36,6 +40,7 @@
 #include "host_combine.hpp"
 #include "host_field.hpp"
 #include "resident.hpp"
+#include "staging.hpp"
 
 #define GM_TRY_S(x)                      \
     do {                                 \
@@ -75,8 +80,9 @@ struct Slot {
     size_t sc_cap = 0;
     uint8_t* pinned = nullptr;
     size_t pinned_cap = 0;
+    gmc::Staging stage;  // pinned staging of the batch's scalars
     bool busy = false;
-    bool launched = false;  // device work queued; the slot's completion thread owns it
+    bool launched = false;  // device work queued by the slot's worker
     bool large = false;     // a batch of the large size class
     bool fail = false;
     std::vector<Request*> reqs;
@@ -85,6 +91,13 @@ struct Slot {
     int curve = 0;
     size_t footprint = 0;
     size_t res_bytes = 0;
+};
+
+// One request's window combine, queued once its batch has left the device.
+struct CombineJob {
+    Request* r = nullptr;
+    int curve = 0, nw = 0, c = 0;
+    std::vector<uint32_t> raw;  // nw XYZZ window sums, device words
 };
 
 // ---- planner MLP (host inference; same network as csrc/mlp.hip)
@@ -140,8 +153,10 @@ struct gm_server {
     const gm_bases* pool[3] = {nullptr, nullptr, nullptr};
     HostMlp mlp;
     std::mutex mu;
-    std::condition_variable cv_sched, cv_done, cv_comp;
+    std::condition_variable cv_sched, cv_done, cv_slot, cv_job;
     std::deque<Request*> queue;
+    std::deque<CombineJob> jobs;
+    int combining = 0;  // jobs taken by a combine thread and not yet published
     std::unordered_map<uint64_t, Request*> table;
     std::vector<Slot> slots;
     size_t inflight_bytes = 0;
@@ -156,7 +171,8 @@ struct gm_server {
     int peak_busy = 0;
     std::vector<double> lat;
     std::thread th_sched;
-    std::vector<std::thread> th_comp;  // one per slot
+    std::vector<std::thread> th_slot;  // one worker per slot
+    std::vector<std::thread> th_comb;  // window combines
 };
 
 namespace {
@@ -181,32 +197,52 @@ bool plan_batch(gm_server* S, const std::vector<Request*>& rs, gm::Plan& p, std:
     return gm::make_plan(p, desc.data(), rs[0]->curve, rs[0]->limbs, rs[0]->bits, sp.data(), (int)rs.size(), exact);
 }
 
-hipError_t grow(uint8_t*& ptr, size_t& cap, size_t want, bool pinned) {
+// Slot buffers grow by a quarter beyond the request. Device buffers are freed
+// and taken stream-ordered on the slot's stream, so a growing slot does not
+// synchronise the device (and stall the other slots' batches).
+hipError_t grow_dev(uint8_t*& ptr, size_t& cap, size_t want, hipStream_t s) {
     if (want <= cap) return hipSuccess;
     if (ptr) {
-        if (pinned)
-            hipHostFree(ptr);
-        else
-            hipFree(ptr);
+        GM_TRY_S(hipFreeAsync(ptr, s));
         ptr = nullptr;
         cap = 0;
     }
     want += want / 4;
-    hipError_t e = pinned ? hipHostMalloc((void**)&ptr, want, hipHostMallocDefault) : hipMalloc(&ptr, want);
-    if (e == hipSuccess) cap = want;
+    const hipError_t e = hipMallocAsync((void**)&ptr, want, s);
+    if (e == hipSuccess)
+        cap = want;
+    else
+        ptr = nullptr;
+    return e;
+}
+hipError_t grow_pinned(uint8_t*& ptr, size_t& cap, size_t want) {
+    if (want <= cap) return hipSuccess;
+    if (ptr) {
+        hipHostFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+    }
+    want += want / 4;
+    const hipError_t e = hipHostMalloc((void**)&ptr, want, hipHostMallocDefault);
+    if (e == hipSuccess)
+        cap = want;
+    else
+        ptr = nullptr;
     return e;
 }
 
 template <class O>
 hipError_t launch_slot(gm_server* S, Slot& sl) {
     const gm_bases* b = S->pool[sl.curve];
+    std::vector<gmc::H2D> cp;
+    cp.reserve(sl.reqs.size());
     uint8_t* sc = sl.d_sc;
     for (auto* r : sl.reqs) {
         const size_t bytes = (size_t)r->n * r->limbs * 8;
-        GM_TRY_S(hipMemcpyAsync(sc, r->borrowed ? r->borrowed : r->scalars.data(), bytes, hipMemcpyHostToDevice,
-                                sl.stream));
+        cp.push_back({sc, r->borrowed ? r->borrowed : r->scalars.data(), bytes});
         sc += bytes;
     }
+    GM_TRY_S(gmc::h2d_staged(sl.stage, (int)S->cfg.gpu_index, sl.stream, cp.data(), (int)cp.size()));
     GM_TRY_S(hipMemcpyAsync(sl.ws + sl.p.o_desc, sl.pinned + sl.res_bytes + 256, sl.desc.size() * sizeof(gm::MsmDesc),
                             hipMemcpyHostToDevice, sl.stream));
     GM_TRY_S(gm::Engine<O>::run(sl.stream, sl.p, reinterpret_cast<const Aff<O>*>(b->d),
@@ -335,51 +371,67 @@ void scheduler(gm_server* S) {
         int busy = 0;
         for (auto& s2 : S->slots) busy += s2.busy;
         S->peak_busy = std::max(S->peak_busy, busy);
-        lk.unlock();
-
-        // device work outside the lock
-        sl.res_bytes = (size_t)p.win_total * gm::xyzz_bytes(sl.curve);
-        size_t sc_bytes = 0;
-        for (auto* r : take) sc_bytes += (size_t)r->n * r->limbs * 8;
-        hipError_t e = grow(sl.ws, sl.ws_cap, p.total, false);
-        if (e == hipSuccess) e = grow(sl.d_sc, sl.sc_cap, sc_bytes, false);
-        if (e == hipSuccess) e = grow(sl.pinned, sl.pinned_cap, sl.res_bytes + 256 + desc.size() * sizeof(gm::MsmDesc),
-                                      true);
-        if (e == hipSuccess) {
-            memcpy(sl.pinned + sl.res_bytes + 256, sl.desc.data(), sl.desc.size() * sizeof(gm::MsmDesc));
-            if (sl.fail)
-                e = hipEventRecord(sl.ev, sl.stream);  // injected failure: nothing is launched
-            else
-                e = sl.curve == GM_CURVE_G1 ? launch_slot<FpOps>(S, sl) : launch_slot<Fp2Ops>(S, sl);
-        }
-        if (e != hipSuccess) {
-            // Copies and kernels of this batch may already be queued on the
-            // slot's stream: the slot (and its HBM share) is released only
-            // after they have drained, and the completer must not wait on the
-            // event of the slot's previous batch.
-            sl.fail = true;
-            (void)hipGetLastError();
-            if (hipEventRecord(sl.ev, sl.stream) != hipSuccess) (void)hipStreamSynchronize(sl.stream);
-        }
-        lk.lock();
-        sl.launched = true;
-        S->batches += 1;
-        S->batched_msms += take.size();
-        S->cv_comp.notify_all();
+        // the slot's worker stages, launches and completes the batch
+        S->cv_slot.notify_all();
     }
 }
 
-// Completion of one slot's batches, in whatever order the slots finish.
-void completer(gm_server* S, int sid) {
+// Publishes one finished request (caller holds S->mu).
+void publish(gm_server* S, Request* r, int rc, double t) {
+    r->rc = rc;
+    r->done = true;
+    r->t_done = t;
+    r->scalars.clear();
+    r->scalars.shrink_to_fit();
+    S->completed += 1;
+    if (rc == 0)
+        S->lat.push_back(t - r->t_submit);
+    else
+        S->failed += 1;
+}
+
+// Device side of one batch on its slot: buffers, staged scalars, pipeline,
+// window sums back to the slot's pinned buffer, the slot's event.
+hipError_t launch_batch(gm_server* S, Slot& sl) {
+    if (sl.fail) return hipEventRecord(sl.ev, sl.stream);  // injected failure: nothing is launched
+    size_t sc_bytes = 0;
+    for (auto* r : sl.reqs) sc_bytes += (size_t)r->n * r->limbs * 8;
+    GM_TRY_S(grow_dev(sl.ws, sl.ws_cap, sl.p.total, sl.stream));
+    GM_TRY_S(grow_dev(sl.d_sc, sl.sc_cap, sc_bytes, sl.stream));
+    GM_TRY_S(grow_pinned(sl.pinned, sl.pinned_cap, sl.res_bytes + 256 + sl.desc.size() * sizeof(gm::MsmDesc)));
+    memcpy(sl.pinned + sl.res_bytes + 256, sl.desc.data(), sl.desc.size() * sizeof(gm::MsmDesc));
+    return sl.curve == GM_CURVE_G1 ? launch_slot<FpOps>(S, sl) : launch_slot<Fp2Ops>(S, sl);
+}
+
+// Worker of one slot: launches each batch the scheduler assigns to it, waits
+// for the device, releases the slot and hands the window combines to the
+// combine threads, in whatever order the slots finish.
+void slot_worker(gm_server* S, int sid) {
     hipSetDevice((int)S->cfg.gpu_index);
     Slot& sl = S->slots[sid];
     for (;;) {
         {
             std::unique_lock<std::mutex> lk(S->mu);
-            S->cv_comp.wait(lk, [&] { return S->stop || (sl.busy && sl.launched); });
-            if (!(sl.busy && sl.launched)) return;  // stop with nothing in flight on this slot
+            S->cv_slot.wait(lk, [&] { return S->stop || (sl.busy && !sl.launched); });
+            if (!(sl.busy && !sl.launched)) return;  // stop with nothing assigned to this slot
         }
-        hipError_t e = hipEventSynchronize(sl.ev);
+        sl.res_bytes = (size_t)sl.p.win_total * gm::xyzz_bytes(sl.curve);
+        if (launch_batch(S, sl) != hipSuccess) {
+            // Copies and kernels of this batch may already be queued on the
+            // slot's stream: the slot (and its HBM share) is released only
+            // after they have drained, never on the event of the slot's
+            // previous batch.
+            sl.fail = true;
+            (void)hipGetLastError();
+            if (hipEventRecord(sl.ev, sl.stream) != hipSuccess) (void)hipStreamSynchronize(sl.stream);
+        }
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            sl.launched = true;
+            S->batches += 1;
+            S->batched_msms += sl.reqs.size();
+        }
+        const hipError_t e = hipEventSynchronize(sl.ev);
         int rc = 0;
         if (sl.fail || e != hipSuccess) {
             rc = -4;
@@ -388,56 +440,62 @@ void completer(gm_server* S, int sid) {
             memcpy(&err, sl.pinned + sl.res_bytes, 4);
             if (err) rc = -5;
         }
-        const int count = (int)sl.reqs.size();
+        std::vector<CombineJob> jobs;
         if (rc == 0) {
             const uint32_t* raw = reinterpret_cast<const uint32_t*>(sl.pinned);
-            auto work = [&](int j0, int j1) {
-                for (int j = j0; j < j1; ++j) {
-                    const gm::MsmDesc& d = sl.desc[j];
-                    if (sl.curve == GM_CURVE_G1) {
-                        gmh::Fp x, y;
-                        const bool f = gmc::host_combine<gmh::Fp>(x, y, raw + (size_t)d.win_base * 64, (int)d.nw, (int)d.c);
-                        gmc::write_affine_abi<gmh::Fp>(sl.reqs[j]->result, f, x, y);
-                    } else {
-                        gmh::Fp2 x, y;
-                        const bool f = gmc::host_combine<gmh::Fp2>(x, y, raw + (size_t)d.win_base * 128, (int)d.nw, (int)d.c);
-                        gmc::write_affine_abi<gmh::Fp2>(sl.reqs[j]->result, f, x, y);
-                    }
-                }
-            };
-            const int nt = count < 4 ? 1 : std::min(count, 8);
-            if (nt == 1) {
-                work(0, count);
-            } else {
-                std::vector<std::thread> th;
-                const int per = (count + nt - 1) / nt;
-                for (int t = 0; t < nt; ++t) {
-                    const int j0 = t * per, j1 = std::min(count, j0 + per);
-                    if (j0 < j1) th.emplace_back(work, j0, j1);
-                }
-                for (auto& t : th) t.join();
+            const size_t words = gm::xyzz_bytes(sl.curve) / 4;
+            jobs.resize(sl.reqs.size());
+            for (size_t j = 0; j < sl.reqs.size(); ++j) {
+                const gm::MsmDesc& d = sl.desc[j];
+                CombineJob& job = jobs[j];
+                job.r = sl.reqs[j];
+                job.curve = sl.curve;
+                job.nw = (int)d.nw;
+                job.c = (int)d.c;
+                job.raw.assign(raw + (size_t)d.win_base * words, raw + (size_t)(d.win_base + d.nw) * words);
             }
         }
         const double t = now_ms();
         std::lock_guard<std::mutex> lk(S->mu);
-        for (auto* r : sl.reqs) {
-            r->rc = rc;
-            r->done = true;
-            r->t_done = t;
-            r->scalars.clear();
-            r->scalars.shrink_to_fit();
-            S->completed += 1;
-            if (rc == 0)
-                S->lat.push_back(t - r->t_submit);
-            else
-                S->failed += 1;
-        }
+        if (rc != 0)
+            for (auto* r : sl.reqs) publish(S, r, rc, t);
+        for (auto& job : jobs) S->jobs.push_back(std::move(job));
         sl.reqs.clear();
         sl.busy = false;
         sl.launched = false;
         S->inflight_bytes -= sl.footprint;
+        S->cv_job.notify_all();
         S->cv_done.notify_all();
         S->cv_sched.notify_one();
+    }
+}
+
+// Combine thread: Horner window combine of one request at a time.
+void combiner(gm_server* S) {
+    for (;;) {
+        CombineJob job;
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            S->cv_job.wait(lk, [&] { return S->stop || !S->jobs.empty(); });
+            if (S->jobs.empty()) return;  // stop with nothing left to combine
+            job = std::move(S->jobs.front());
+            S->jobs.pop_front();
+            S->combining += 1;
+        }
+        if (job.curve == GM_CURVE_G1) {
+            gmh::Fp x, y;
+            const bool f = gmc::host_combine<gmh::Fp>(x, y, job.raw.data(), job.nw, job.c);
+            gmc::write_affine_abi<gmh::Fp>(job.r->result, f, x, y);
+        } else {
+            gmh::Fp2 x, y;
+            const bool f = gmc::host_combine<gmh::Fp2>(x, y, job.raw.data(), job.nw, job.c);
+            gmc::write_affine_abi<gmh::Fp2>(job.r->result, f, x, y);
+        }
+        const double t = now_ms();
+        std::lock_guard<std::mutex> lk(S->mu);
+        publish(S, job.r, 0, t);
+        S->combining -= 1;
+        S->cv_done.notify_all();
     }
 }
 
@@ -483,7 +541,9 @@ int gm_server_create(gm_server** out, const gm_server_config* cfg, const gm_base
     else if (cfg->small_max_n < 0)
         S->small_max_n = 0;
     S->th_sched = std::thread(scheduler, S);
-    for (int i = 0; i < cfg->slots; ++i) S->th_comp.emplace_back(completer, S, i);
+    for (int i = 0; i < cfg->slots; ++i) S->th_slot.emplace_back(slot_worker, S, i);
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    for (unsigned i = 0; i < std::min(8u, hw); ++i) S->th_comb.emplace_back(combiner, S);
     *out = S;
     return 0;
 }
@@ -567,28 +627,34 @@ void gm_server_destroy(gm_server* S) {
     if (!S) return;
     {
         std::unique_lock<std::mutex> lk(S->mu);
-        // drain: wait until queued and in-flight work has completed
+        // drain: wait until queued, in-flight and combining work has completed
         S->cv_done.wait(lk, [&] {
-            if (!S->queue.empty()) return false;
+            if (!S->queue.empty() || !S->jobs.empty() || S->combining > 0) return false;
             for (auto& sl : S->slots)
                 if (sl.busy) return false;
             return true;
         });
         S->stop = true;
         S->cv_sched.notify_all();
-        S->cv_comp.notify_all();
+        S->cv_slot.notify_all();
+        S->cv_job.notify_all();
     }
     if (S->th_sched.joinable()) S->th_sched.join();
-    for (auto& th : S->th_comp)
+    for (auto& th : S->th_slot)
+        if (th.joinable()) th.join();
+    for (auto& th : S->th_comb)
         if (th.joinable()) th.join();
     hipSetDevice((int)S->cfg.gpu_index);
     for (auto& sl : S->slots) {
-        if (sl.stream) hipStreamSynchronize(sl.stream);
+        if (sl.stream) {
+            if (sl.ws) hipFreeAsync(sl.ws, sl.stream);
+            if (sl.d_sc) hipFreeAsync(sl.d_sc, sl.stream);
+            hipStreamSynchronize(sl.stream);
+        }
         if (sl.ev) hipEventDestroy(sl.ev);
         if (sl.stream) hipStreamDestroy(sl.stream);
-        if (sl.ws) hipFree(sl.ws);
-        if (sl.d_sc) hipFree(sl.d_sc);
         if (sl.pinned) hipHostFree(sl.pinned);
+        gmc::staging_release(sl.stage);
     }
     for (auto& kv : S->table) delete kv.second;
     delete S;
