@@ -123,6 +123,43 @@ def test_generators_on_curve_and_order():
     assert m.g1_from_abi(ng[0]) == m.ec_neg(m.FP_OPS, m.G1_STD)
 
 
+def _g1_rec(x, y):
+    g = np.zeros(1, dtype=o.G1_DT)
+    g[0]["x"] = o.fp_unop("or_fp_to_mont", L(x % P))
+    g[0]["y"] = o.fp_unop("or_fp_to_mont", L(y % P))
+    return g
+
+
+def _g2_rec(x, y):
+    g = np.zeros(1, dtype=o.G2_DT)
+    g[0]["x"] = np.concatenate([o.fp_unop("or_fp_to_mont", L(v % P)) for v in x])
+    g[0]["y"] = np.concatenate([o.fp_unop("or_fp_to_mont", L(v % P)) for v in y])
+    return g
+
+
+def test_is_on_curve_rejects_points_off_the_curve():
+    """The on-curve tests are not constant: points off y^2 = x^3 + 1 (G1) and
+    y^2 = x^3 + (1 + i) (G2) are rejected, among them a point of the
+    reference's y^2 = x^3 + 4, and a point of the real curve other than the
+    generator is accepted."""
+    lib = o.lib()
+    assert lib.or_g1_is_on_curve(o.ptr(_g1_rec(0, 1)))  # 1 = 0 + 1
+    assert lib.or_g1_is_on_curve(o.ptr(_g1_rec(0, P - 1)))
+    assert not lib.or_g1_is_on_curve(o.ptr(_g1_rec(0, 2)))  # on y^2 = x^3 + 4 only
+    gx, gy = m.G1_STD
+    assert not lib.or_g1_is_on_curve(o.ptr(_g1_rec(gx, gy + 1)))
+    assert not lib.or_g1_is_on_curve(o.ptr(_g1_rec(gx + 1, gy)))
+    (x0, x1), (y0, y1) = m.G2_STD
+    assert lib.or_g2_is_on_curve(o.ptr(_g2_rec((x0, x1), (P - y0, P - y1))))  # -G2
+    assert not lib.or_g2_is_on_curve(o.ptr(_g2_rec((x0, x1), (y0 + 1, y1))))
+    assert not lib.or_g2_is_on_curve(o.ptr(_g2_rec((x0, x1), (y0, y1 + 1))))
+    assert not lib.or_g2_is_on_curve(o.ptr(_g2_rec((x1, x0), (y0, y1))))
+    # infinity records are on the curve whatever their coordinates
+    inf = _g1_rec(0, 2)
+    inf[0]["inf"] = 1
+    assert lib.or_g1_is_on_curve(o.ptr(inf))
+
+
 def test_scalar_mul_against_bigint():
     g1, g2 = o.g1_generator(), o.g2_generator()
     for k in (1, 2, 3, 0xFFFFFFFFFFFFFFFF, m.R_ORDER // 3 + 17):
