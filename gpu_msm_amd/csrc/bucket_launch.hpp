@@ -47,9 +47,10 @@ hipError_t launch_fixup(hipStream_t s, const Plan& p, uint8_t* ws) {
     X* buckets = reinterpret_cast<X*>(ws + p.o_buckets);
     uint32_t* heavy = reinterpret_cast<uint32_t*>(ws + p.o_heavy);
     uint32_t* heavy_count = reinterpret_cast<uint32_t*>(ws + p.o_err) + 1;
+    uint32_t* entries = reinterpret_cast<uint32_t*>(ws + p.o_err) + 2;
     const MsmDesc* desc = reinterpret_cast<const MsmDesc*>(ws + p.o_desc);
     k_bucket_fixup<K><<<blocks_for((long)nbk * Stor<K>::LPI, 64), 64, 0, s>>>(desc, p.count, nbk, off, head, tail,
-                                                                             buckets, heavy, heavy_count);
+                                                                             buckets, heavy, heavy_count, entries);
     GM_TRY(hipGetLastError());
     // one workgroup per queued heavy bucket, grid-stride over the queue
     constexpr int HT = 64 * Stor<K>::LPI;

@@ -155,27 +155,28 @@ int run_batch_locked(DevCtx* c, const Aff<O>* d_bases, const uint64_t* d_scalars
     }
     const size_t desc_b = (size_t)count * sizeof(gm::MsmDesc);
     const size_t res_b = device_combine ? (size_t)count * sizeof(Aff<O>) : (size_t)p.win_total * sizeof(Xyzz<O>);
-    const size_t o_res = gm::al_size(desc_b), o_flags = o_res + gm::al_size(res_b);
-    if (ensure_pinned(c, o_flags + 256) != hipSuccess) return -3;
+    // pinned: the descriptors, then the error words and the results
+    // ERR_BYTES apart, as the workspace holds the error words and the window
+    // sums, so that one copy brings both back
+    const size_t o_flags = gm::al_size(desc_b), o_res = o_flags + gm::ERR_BYTES;
+    if (ensure_pinned(c, o_res + gm::al_size(res_b)) != hipSuccess) return -3;
     uint8_t* h = c->pinned;
     memcpy(h, desc.data(), desc_b);
     if (hipMemcpyAsync(ws + p.o_desc, h, desc_b, hipMemcpyHostToDevice, c->stream) != hipSuccess) return -4;
     if (gm::Engine<O>::run(c->stream, p, d_bases, d_scalars, ws, &c->prof) != hipSuccess) return -4;
     c->prof_valid = c->prof.on;
-    if (hipMemcpyAsync(h + o_flags, ws + p.o_err, 4, hipMemcpyDeviceToHost, c->stream) != hipSuccess) return -4;
-    if (hipMemcpyAsync(h + o_flags + 4, ws + p.o_off + (size_t)p.bkt_total * 4, 4, hipMemcpyDeviceToHost,
-                       c->stream) != hipSuccess)
-        return -4;
     if (device_combine) {
         if (gm::Engine<O>::device_combine(c->stream, p, ws) != hipSuccess) return -4;
+        if (hipMemcpyAsync(h + o_flags, ws + p.o_err, 16, hipMemcpyDeviceToHost, c->stream) != hipSuccess) return -4;
         if (hipMemcpyAsync(h + o_res, ws + p.o_out, res_b, hipMemcpyDeviceToHost, c->stream) != hipSuccess) return -4;
-    } else {
-        if (hipMemcpyAsync(h + o_res, ws + p.o_wsum, res_b, hipMemcpyDeviceToHost, c->stream) != hipSuccess) return -4;
+    } else if (hipMemcpyAsync(h + o_flags, ws + p.o_err, gm::ERR_BYTES + res_b, hipMemcpyDeviceToHost, c->stream) !=
+               hipSuccess) {
+        return -4;
     }
     if (hipStreamSynchronize(c->stream) != hipSuccess) return -4;
     uint32_t err, entries;
     memcpy(&err, h + o_flags, 4);
-    memcpy(&entries, h + o_flags + 4, 4);
+    memcpy(&entries, h + o_flags + 8, 4);
     if (err) return -5;
     c->last_c = (int)desc[0].c;
     c->last_nw = (int)desc[0].nw;
@@ -375,16 +376,17 @@ template <class O>
 hipError_t batch_enqueue(gm_batch* bt) {
     GM_TRY_HOST(gm::Engine<O>::run(bt->stream, bt->p, reinterpret_cast<const Aff<O>*>(bt->b->d), bt->s->d, bt->ws,
                                    nullptr));
+    // pinned: the error words, then the results ERR_BYTES on
     if (bt->device_combine) {
         GM_TRY_HOST(gm::Engine<O>::device_combine(bt->stream, bt->p, bt->ws));
-        GM_TRY_HOST(
-            hipMemcpyAsync(bt->pinned, bt->ws + bt->p.o_out, bt->res_bytes, hipMemcpyDeviceToHost, bt->stream));
+        GM_TRY_HOST(hipMemcpyAsync(bt->pinned + gm::ERR_BYTES, bt->ws + bt->p.o_out, bt->res_bytes,
+                                   hipMemcpyDeviceToHost, bt->stream));
+        GM_TRY_HOST(hipMemcpyAsync(bt->pinned, bt->ws + bt->p.o_err, 16, hipMemcpyDeviceToHost, bt->stream));
     } else {
-        GM_TRY_HOST(
-            hipMemcpyAsync(bt->pinned, bt->ws + bt->p.o_wsum, bt->res_bytes, hipMemcpyDeviceToHost, bt->stream));
+        // error words and window sums in one copy
+        GM_TRY_HOST(hipMemcpyAsync(bt->pinned, bt->ws + bt->p.o_err, gm::ERR_BYTES + bt->res_bytes,
+                                   hipMemcpyDeviceToHost, bt->stream));
     }
-    GM_TRY_HOST(hipMemcpyAsync(bt->pinned + gm::al_size(bt->res_bytes), bt->ws + bt->p.o_err, 4,
-                               hipMemcpyDeviceToHost, bt->stream));
     return hipSuccess;
 }
 
@@ -432,14 +434,15 @@ template <class O>
 int batch_results(gm_batch* bt, void* results_affine) {
     using HF = typename HostOf<O>::HF;
     uint32_t err;
-    memcpy(&err, bt->pinned + gm::al_size(bt->res_bytes), 4);
+    memcpy(&err, bt->pinned, 4);
     if (err) return -5;
+    const uint8_t* res_h = bt->pinned + gm::ERR_BYTES;
     constexpr int W = words_of<HF>();
     const size_t rec = HostOf<O>::CURVE == gm::CURVE_G1 ? 120 : 232;
     std::vector<AffResult<HF>> res((size_t)bt->count);
     if (bt->device_combine) {
         for (int j = 0; j < bt->count; ++j) {
-            const uint32_t* a = reinterpret_cast<const uint32_t*>(bt->pinned + (size_t)j * sizeof(Aff<O>));
+            const uint32_t* a = reinterpret_cast<const uint32_t*>(res_h + (size_t)j * sizeof(Aff<O>));
             res[j].finite = a[15] != gm::INF_MARK;
             if (res[j].finite) {
                 dev_elem_to_host(res[j].x, a);
@@ -447,7 +450,7 @@ int batch_results(gm_batch* bt, void* results_affine) {
             }
         }
     } else {
-        host_combine_many<HF>(res, reinterpret_cast<const uint32_t*>(bt->pinned), bt->desc.data(), bt->count);
+        host_combine_many<HF>(res, reinterpret_cast<const uint32_t*>(res_h), bt->desc.data(), bt->count);
     }
     for (int j = 0; j < bt->count; ++j)
         write_affine_abi<HF>(reinterpret_cast<uint8_t*>(results_affine) + (size_t)j * rec, res[j].finite, res[j].x,
@@ -778,7 +781,8 @@ int gm_batch_create(gm_batch** out, const gm_bases* b, const gm_scalars* s, cons
         return -2;
     }
     if (hipMalloc(&bt->ws, bt->p.total) != hipSuccess ||
-        hipHostMalloc((void**)&bt->pinned, gm::al_size(bt->res_bytes) + 256, hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc((void**)&bt->pinned, gm::ERR_BYTES + gm::al_size(bt->res_bytes), hipHostMallocDefault) !=
+            hipSuccess) {
         batch_release(bt);
         return -3;
     }

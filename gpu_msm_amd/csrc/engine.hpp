@@ -65,6 +65,11 @@ struct MsmDesc {
 };
 static_assert(sizeof(MsmDesc) == 80, "MsmDesc layout");
 
+// Bytes from the error words (error flag, heavy-bucket count, entry count)
+// to the window sums in a plan's workspace: one device-to-host copy of
+// ERR_BYTES + the window sums brings the flags and the sums together.
+constexpr size_t ERR_BYTES = 256;
+
 // Segments summed by one window-sum tile (64 threads x 4) before the
 // per-window total over tiles.
 constexpr uint32_t WSUM_TILE = 256;

@@ -378,7 +378,8 @@ __device__ __forceinline__ void bucket_lanes(const MsmDesc& m, const uint32_t* _
 // tail[first lane] + head[every later lane it reaches]. A bucket cut across
 // more than HEAVY_SPAN chunks (few, very full buckets: a top window with few
 // bits, or repeated scalars) is queued for k_bucket_fixup_heavy instead of
-// being summed serially by one lane.
+// being summed serially by one lane. The first lane also copies the batch's
+// entry count (the scan's total) next to the error flag for the host.
 constexpr uint32_t HEAVY_SPAN = 8;
 
 template <class O>
@@ -386,8 +387,10 @@ __global__ void __launch_bounds__(64) k_bucket_fixup(const MsmDesc* __restrict__
                                                      const uint32_t* __restrict__ off,
                                                      const typename Stor<O>::X* __restrict__ head,
                                                      const typename Stor<O>::X* __restrict__ tail, typename Stor<O>::X* __restrict__ buckets,
-                                                     uint32_t* __restrict__ heavy, uint32_t* __restrict__ heavy_count) {
+                                                     uint32_t* __restrict__ heavy, uint32_t* __restrict__ heavy_count,
+                                                     uint32_t* __restrict__ entries) {
     const int g = (blockIdx.x * blockDim.x + threadIdx.x) / Stor<O>::LPI;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *entries = off[nbk];
     if (g >= nbk) return;
     const uint32_t s = off[g], e = off[g + 1];
     Xyzz<O> acc;

@@ -254,9 +254,12 @@ bool make_plan(Plan& p, MsmDesc* d, int curve, int L, int bits, const MsmSpec* s
     p.o_heavy = take((bkt + 1) * 4);
     p.o_segs = take(seg * xb);
     p.o_tiles = take(tile * xb);
+    // error flag, heavy-bucket count and entry count, right before the window
+    // sums (ERR_BYTES apart) so that one copy brings both to the host
+    p.o_err = take(16);
     p.o_wsum = take(win * xb);
     p.o_out = take((size_t)count * aff_bytes(curve));
-    p.o_err = take(8);  // error flag, heavy-bucket count
+    if (p.o_wsum != p.o_err + ERR_BYTES) return false;
     p.o_cub = take(cub);
     p.total = o;
     return true;

@@ -231,6 +231,10 @@ hipError_t grow_pinned(uint8_t*& ptr, size_t& cap, size_t want) {
     return e;
 }
 
+// The slot's pinned buffer: the error words, the window sums ERR_BYTES on,
+// then the batch's descriptors.
+size_t desc_at(const Slot& sl) { return gm::ERR_BYTES + gm::al_size(sl.res_bytes); }
+
 template <class O>
 hipError_t launch_slot(gm_server* S, Slot& sl) {
     const gm_bases* b = S->pool[sl.curve];
@@ -243,12 +247,13 @@ hipError_t launch_slot(gm_server* S, Slot& sl) {
         sc += bytes;
     }
     GM_TRY_S(gmc::h2d_staged(sl.stage, (int)S->cfg.gpu_index, sl.stream, cp.data(), (int)cp.size()));
-    GM_TRY_S(hipMemcpyAsync(sl.ws + sl.p.o_desc, sl.pinned + sl.res_bytes + 256, sl.desc.size() * sizeof(gm::MsmDesc),
+    GM_TRY_S(hipMemcpyAsync(sl.ws + sl.p.o_desc, sl.pinned + desc_at(sl), sl.desc.size() * sizeof(gm::MsmDesc),
                             hipMemcpyHostToDevice, sl.stream));
     GM_TRY_S(gm::Engine<O>::run(sl.stream, sl.p, reinterpret_cast<const Aff<O>*>(b->d),
                                 reinterpret_cast<const uint64_t*>(sl.d_sc), sl.ws, nullptr));
-    GM_TRY_S(hipMemcpyAsync(sl.pinned, sl.ws + sl.p.o_wsum, sl.res_bytes, hipMemcpyDeviceToHost, sl.stream));
-    GM_TRY_S(hipMemcpyAsync(sl.pinned + sl.res_bytes, sl.ws + sl.p.o_err, 4, hipMemcpyDeviceToHost, sl.stream));
+    // error words and window sums in one copy (ERR_BYTES apart in the workspace)
+    GM_TRY_S(hipMemcpyAsync(sl.pinned, sl.ws + sl.p.o_err, gm::ERR_BYTES + sl.res_bytes, hipMemcpyDeviceToHost,
+                            sl.stream));
     return hipEventRecord(sl.ev, sl.stream);
 }
 
@@ -398,8 +403,8 @@ hipError_t launch_batch(gm_server* S, Slot& sl) {
     for (auto* r : sl.reqs) sc_bytes += (size_t)r->n * r->limbs * 8;
     GM_TRY_S(grow_dev(sl.ws, sl.ws_cap, sl.p.total, sl.stream));
     GM_TRY_S(grow_dev(sl.d_sc, sl.sc_cap, sc_bytes, sl.stream));
-    GM_TRY_S(grow_pinned(sl.pinned, sl.pinned_cap, sl.res_bytes + 256 + sl.desc.size() * sizeof(gm::MsmDesc)));
-    memcpy(sl.pinned + sl.res_bytes + 256, sl.desc.data(), sl.desc.size() * sizeof(gm::MsmDesc));
+    GM_TRY_S(grow_pinned(sl.pinned, sl.pinned_cap, desc_at(sl) + sl.desc.size() * sizeof(gm::MsmDesc)));
+    memcpy(sl.pinned + desc_at(sl), sl.desc.data(), sl.desc.size() * sizeof(gm::MsmDesc));
     return sl.curve == GM_CURVE_G1 ? launch_slot<FpOps>(S, sl) : launch_slot<Fp2Ops>(S, sl);
 }
 
@@ -437,12 +442,12 @@ void slot_worker(gm_server* S, int sid) {
             rc = -4;
         } else {
             uint32_t err;
-            memcpy(&err, sl.pinned + sl.res_bytes, 4);
+            memcpy(&err, sl.pinned, 4);
             if (err) rc = -5;
         }
         std::vector<CombineJob> jobs;
         if (rc == 0) {
-            const uint32_t* raw = reinterpret_cast<const uint32_t*>(sl.pinned);
+            const uint32_t* raw = reinterpret_cast<const uint32_t*>(sl.pinned + gm::ERR_BYTES);
             const size_t words = gm::xyzz_bytes(sl.curve) / 4;
             jobs.resize(sl.reqs.size());
             for (size_t j = 0; j < sl.reqs.size(); ++j) {

@@ -2,6 +2,8 @@
 // scatter into bucket-contiguous order (curve independent).
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
+
 #include "engine_impl.hpp"
 #include "sort_kernels.hpp"
 
@@ -24,11 +26,6 @@ hipError_t launch_sort(hipStream_t s, const Plan& p, const uint64_t* d_scalars, 
     auto mark = [&](int st) -> hipError_t {
         return (prof && prof->on) ? hipEventRecord(prof->ev[st], s) : hipSuccess;
     };
-    // Counts (+1 sentinel so the scan also yields the total) and the error
-    // flag are cleared by a fill kernel, not hipMemsetAsync, so the sequence
-    // can be captured in a hipGraph and replayed.
-    GM_TRY(launch_fill_u32(s, cnt, nbk + 1, 0));
-    GM_TRY(launch_fill_u32(s, err, 2, 0));  // error flag and heavy-bucket count
     const bool tiled = p.lds_B > 0;
     const size_t lds = (size_t)p.lds_B * 4;
     uint16_t* dig = reinterpret_cast<uint16_t*>(ws + p.o_dig);
@@ -37,7 +34,17 @@ hipError_t launch_sort(hipStream_t s, const Plan& p, const uint64_t* d_scalars, 
     uint32_t* qbase = reinterpret_cast<uint32_t*>(ws + p.o_qbase);
     uint64_t* tmp = reinterpret_cast<uint64_t*>(ws + p.o_tmp);
     const uint32_t nblocks = (uint32_t)((nbk + SORT_BLOCK - 1) / SORT_BLOCK);
-    if (p.qmax) GM_TRY(launch_fill_u32(s, blkcnt, nblocks, 0));
+    // Counts (+1 sentinel so the scan also yields the total), the error words
+    // (error flag, heavy-bucket count, entry count) and the block counts of
+    // the two-pass placement are cleared by one fill kernel, not
+    // hipMemsetAsync, so the sequence can be captured in a hipGraph and
+    // replayed.
+    {
+        const size_t nblk = p.qmax ? nblocks : 0;
+        const size_t longest = std::max<size_t>(nbk + 1, std::max<size_t>(4, nblk));
+        k_zero3<<<(unsigned)((longest + 255) / 256), 256, 0, s>>>(cnt, nbk + 1, err, 4, blkcnt, nblk);
+        GM_TRY(hipGetLastError());
+    }
     if (tiled && lds > 65536) {
         // above the default dynamic LDS limit; once per process per kernel
         static const hipError_t attr = [] {

@@ -400,4 +400,13 @@ __global__ void __launch_bounds__(256) k_fill_u32(uint32_t* __restrict__ p, size
     if (i < n) p[i] = v;
 }
 
+// Zeroes three word ranges in one launch (the grid covers the longest).
+__global__ void __launch_bounds__(256) k_zero3(uint32_t* __restrict__ a, size_t na, uint32_t* __restrict__ b,
+                                               size_t nb, uint32_t* __restrict__ c, size_t nc) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < na) a[i] = 0;
+    if (i < nb) b[i] = 0;
+    if (i < nc) c[i] = 0;
+}
+
 }  // namespace gm
