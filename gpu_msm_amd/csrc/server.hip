@@ -464,7 +464,12 @@ void slot_worker(gm_server* S, int sid) {
         std::lock_guard<std::mutex> lk(S->mu);
         if (rc != 0)
             for (auto* r : sl.reqs) publish(S, r, rc, t);
-        for (auto& job : jobs) S->jobs.push_back(std::move(job));
+        // the small class's combines go first, ahead of queued large ones
+        if (sl.large) {
+            for (auto& job : jobs) S->jobs.push_back(std::move(job));
+        } else {
+            for (auto it = jobs.rbegin(); it != jobs.rend(); ++it) S->jobs.push_front(std::move(*it));
+        }
         sl.reqs.clear();
         sl.busy = false;
         sl.launched = false;
