@@ -125,8 +125,9 @@ int gm_mlp_train(float* params, const float* x, const float* t, int n, int epoch
  * A scheduler thread forms batches from the head of the queue (one curve per
  * batch) whenever one of `slots` stream slots is free, admitting requests
  * while the batch's device footprint fits the HBM budget left by in-flight
- * batches; batches on different slots overlap on the device; a completion
- * thread finishes window combines on host threads. */
+ * batches; batches on different slots overlap on the device; a worker per
+ * slot stages each batch's scalars and frees the slot when the device is done,
+ * and a pool of host threads finishes the window combines. */
 #define GM_PLANNER_HEURISTIC 0
 #define GM_PLANNER_MLP 1
 typedef struct gm_server_config {
