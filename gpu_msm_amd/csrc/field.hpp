@@ -1033,8 +1033,9 @@ struct Fp2Ops {
 // wave carry components c0 and c1 of the same Fp2 values, so each lane holds
 // half the state of a point (a G2 XYZZ point is 64 registers per lane instead
 // of 128), which keeps the mixed addition below 256 registers and two waves
-// per SIMD in flight. A product swaps the partner's operand components with a
-// DPP quad permutation and each lane reduces its own component:
+// per SIMD in flight. A product reads the partner's component of one operand
+// and both components of the other through DPP quad permutations, and each
+// lane reduces its own component:
 //   part 0: a0 b0 - a1 b1,   part 1: a1 b0 + a0 b1,
 // both as x y + u w with signed w (w = -b1 for part 0, +b1 for part 1), one
 // fused column sum and one Montgomery reduction per lane; the result is
@@ -1054,28 +1055,39 @@ __device__ __forceinline__ void fp_pair_swap(Fp& r, const Fp& a) {
     for (int i = 0; i < NL; ++i) r.v[i] = pair_swap(a.v[i]);
 }
 
+// Value of v in the pair's part-0 lane (quad_perm [0, 0, 2, 2]) and in its
+// part-1 lane ([1, 1, 3, 3]), given to both lanes of the pair. pair_part1
+// is written as an update with 0 as the old value, so the compiler folds
+// the read into a following xor or add (v_xor_b32_dpp, v_add_u32_dpp).
+__device__ __forceinline__ uint32_t pair_part0(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xA0, 0xF, 0xF, false);
+}
+__device__ __forceinline__ uint32_t pair_part1(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xF5, 0xF, 0xF, true);
+}
+
 // Signed limbs of the operands a lane multiplies for its own component of
 // (a0 + a1 i)(b0 + b1 i) = (a0 b0 - a1 b1) + (a1 b0 + a0 b1) i, as
-// a y + ao w with ao the partner's part of a:
-//   part 0: y = b0 (own), w = -b1 (partner);  part 1: y = b0 (partner), w = +b1 (own).
-// `neg` negates both (for a subtracted product).
-__device__ __forceinline__ void lane_operands(int32_t (&y)[NL], int32_t (&w)[NL], const Fp& b, const Fp& bo,
-                                              bool neg) {
-    const bool q = lane_part() != 0;
+// a y + ao w with ao the partner's part of a: y = b0 in both lanes, w = -b1
+// in part 0 and +b1 in part 1, each read from its lane of the pair. The sign
+// is (v ^ s) - s with s = -1 where w is negated. `neg` negates both (for a
+// subtracted product): y = -b0, w = +b1 in part 0 and -b1 in part 1.
+__device__ __forceinline__ void lane_operands(int32_t (&y)[NL], int32_t (&w)[NL], const Fp& b, bool neg) {
+    int32_t s = ((lane_part() != 0) == neg) ? -1 : 0;
+    asm("" : "+v"(s));  // keeps the xor and subtract, where a select would not take the DPP read
 #pragma unroll
     for (int i = 0; i < NL; ++i) {
-        const int32_t yv = (int32_t)(q ? bo.v[i] : b.v[i]);
-        const int32_t wv = (int32_t)(q ? b.v[i] : bo.v[i]);
-        y[i] = neg ? -yv : yv;
-        w[i] = (q != neg) ? wv : -wv;
+        const int32_t b0 = (int32_t)pair_part0(b.v[i]);
+        y[i] = neg ? -b0 : b0;
+        w[i] = ((int32_t)pair_part1(b.v[i]) ^ s) - s;
     }
 }
 
-// Own component of a b; ao / bo are the partner's parts.
+// Own component of a b; ao is the partner's part of a.
 template <bool CHAIN = false>
-__device__ __forceinline__ void fp2_lane_mul(Fp& r, const Fp& a, const Fp& ao, const Fp& b, const Fp& bo) {
+__device__ __forceinline__ void fp2_lane_mul(Fp& r, const Fp& a, const Fp& ao, const Fp& b) {
     int32_t y[NL], w[NL];
-    lane_operands(y, w, b, bo, false);
+    lane_operands(y, w, b, false);
     const Fp* const x[2] = {&a, &ao};
     const int32_t(*const yy[2])[NL] = {&y, &w};
     uint32_t t[NL];
@@ -1090,12 +1102,11 @@ __device__ __forceinline__ void fp2_lane_mul(Fp& r, const Fp& a, const Fp& ao, c
 // reduction; with a and b raw (magnitude below 2p) in (-16p^2, 16p^2) and
 // (-4p, 5p), which `wide` folds.
 template <bool CHAIN = false>
-__device__ __forceinline__ void fp2_lane_mul_sub(Fp& r, const Fp& a, const Fp& ao, const Fp& b, const Fp& bo,
-                                                 const Fp& c, const Fp& co, const Fp& d, const Fp& dox,
-                                                 bool wide = false) {
+__device__ __forceinline__ void fp2_lane_mul_sub(Fp& r, const Fp& a, const Fp& ao, const Fp& b, const Fp& c,
+                                                 const Fp& co, const Fp& d, bool wide = false) {
     int32_t y0[NL], w0[NL], y1[NL], w1[NL];
-    lane_operands(y0, w0, b, bo, false);
-    lane_operands(y1, w1, d, dox, true);
+    lane_operands(y0, w0, b, false);
+    lane_operands(y1, w1, d, true);
     const Fp* const x[4] = {&a, &ao, &c, &co};
     const int32_t(*const yy[4])[NL] = {&y0, &w0, &y1, &w1};
     uint32_t t[NL];
@@ -1114,15 +1125,15 @@ __device__ __forceinline__ void fp2_lane_mul_sub(Fp& r, const Fp& a, const Fp& a
 // lies in (-2p, 3p); returned as t / top before the fold.
 template <bool CHAIN = false>
 __device__ __forceinline__ void fp2_lane_sqr_cols(uint32_t (&t)[NL], int64_t& top, const Fp& a) {
-    Fp ao, x;
-    fp_pair_swap(ao, a);
-    const bool q = lane_part() != 0;
+    Fp x;
+    // part 0: x = a0 + a1, y = a0 - a1; part 1: x = a1 + a1, y = a0
+    const int32_t m = lane_part() != 0 ? 0 : -1;
     int32_t y[NL];
 #pragma unroll
     for (int i = 0; i < NL; ++i) {
-        const int32_t av = (int32_t)a.v[i], bv = (int32_t)ao.v[i];
-        x.v[i] = (uint32_t)(av + (q ? av : bv));
-        y[i] = q ? bv : av - bv;
+        const int32_t a1 = (int32_t)pair_part1(a.v[i]);
+        x.v[i] = (uint32_t)((int32_t)a.v[i] + a1);
+        y[i] = (int32_t)pair_part0(a.v[i]) - (a1 & m);
     }
     const Fp* const xs[1] = {&x};
     const int32_t(*const ys[1])[NL] = {&y};
@@ -1141,10 +1152,9 @@ struct Fp2LaneOpsT {
     __device__ static __forceinline__ void dbl(F& r, const F& a) { fp_dbl(r, a); }
     __device__ static __forceinline__ void neg(F& r, const F& a) { fp_neg(r, a); }
     __device__ static __forceinline__ void mul(F& r, const F& a, const F& b) {
-        Fp ao, bo;
+        Fp ao;
         fp_pair_swap(ao, a);
-        fp_pair_swap(bo, b);
-        fp2_lane_mul<CHAIN>(r, a, ao, b, bo);
+        fp2_lane_mul<CHAIN>(r, a, ao, b);
     }
     // (a0 + a1 i)^2 = (a0 + a1)(a0 - a1) + 2 a0 a1 i: one plain product per
     // lane (part 0: (a0 + a1)(a0 - a1), part 1: (a1 + a1) a0), operands
@@ -1162,12 +1172,10 @@ struct Fp2LaneOpsT {
         fp_mul(r, x, y);
     }
     __device__ static __forceinline__ void mul_sub(F& r, const F& a, const F& b, const F& c, const F& d) {
-        Fp ao, bo, co, dx;
+        Fp ao, co;
         fp_pair_swap(ao, a);
-        fp_pair_swap(bo, b);
         fp_pair_swap(co, c);
-        fp_pair_swap(dx, d);
-        fp2_lane_mul_sub<CHAIN>(r, a, ao, b, bo, c, co, d, dx);
+        fp2_lane_mul_sub<CHAIN>(r, a, ao, b, c, co, d);
     }
     // Raw differences, as FpOps has them: a difference that only feeds
     // products stays limb by limb (signed limbs, value of magnitude below
@@ -1183,12 +1191,10 @@ struct Fp2LaneOpsT {
     }
     __device__ static __forceinline__ void mul_raw(F& r, const F& a, const F& b) { mul(r, a, b); }
     __device__ static __forceinline__ void mul_sub_raw(F& r, const F& a, const F& b, const F& c, const F& d) {
-        Fp ao, bo, co, dx;
+        Fp ao, co;
         fp_pair_swap(ao, a);
-        fp_pair_swap(bo, b);
         fp_pair_swap(co, c);
-        fp_pair_swap(dx, d);
-        fp2_lane_mul_sub<CHAIN>(r, a, ao, b, bo, c, co, d, dx, true);
+        fp2_lane_mul_sub<CHAIN>(r, a, ao, b, c, co, d, true);
     }
     // Own component of a^2 - b - 2c (a raw, b and c in [0, 2p)): the square
     // reduces to (-2p, 3p); 8p - b - 2c (limbs 8 p_i - b_i - 2 c_i, in
