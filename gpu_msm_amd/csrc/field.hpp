@@ -361,22 +361,27 @@ __device__ __forceinline__ void col_prods(int64_t& acc, const Fp* const (&x)[T],
         col_prods<T, K, J0 + N>(acc, x, y);
     }
 }
-// Reduction products m[i] * p[K - i] of column K for i in [I0, I1).
-template <int K, int I0, size_t... Is>
-__device__ __forceinline__ void col_red_blk(int64_t& acc, const uint32_t (&m)[NL], std::index_sequence<Is...>) {
+// Reduction products m[i] * p[K - i] of column K for i in [I0, I1) (signed
+// mads: an m[i] of -1 subtracts).
+template <int K, int I0, int NM, size_t... Is>
+__device__ __forceinline__ void col_red_blk(int64_t& acc, const uint32_t (&m)[NM], std::index_sequence<Is...>) {
     mad_blk<true>(acc, (int32_t)m[I0 + (int)Is]..., (int32_t)PL[K - I0 - (int)Is]...);
 }
-template <int K, int I0, int I1>
-__device__ __forceinline__ void col_reds(int64_t& acc, const uint32_t (&m)[NL]) {
+template <int K, int I0, int I1, int NM>
+__device__ __forceinline__ void col_reds(int64_t& acc, const uint32_t (&m)[NM]) {
     if constexpr (I0 < I1) {
         constexpr int N = I1 - I0 < 8 ? I1 - I0 : 8;
         col_red_blk<K, I0>(acc, m, std::make_index_sequence<N>{});
         col_reds<K, I0 + N, I1>(acc, m);
     }
 }
-// The single-chain Montgomery product scan, one column per step.
-template <int T, int K>
-__device__ __forceinline__ void redc_chain_cols(uint32_t (&t)[NL], int64_t& acc, uint32_t (&m)[NL],
+// The single-chain Montgomery product scan, one column per step. With OFFS,
+// m[NL] (0 or -1, set by the caller) is one more row of the reduction at
+// weight R = 2^448: the high columns take m[NL] * p[K - NL], which subtracts
+// p R from the sum (the reduced value drops by p) where m[NL] = -1; its top
+// limb p[NL - 1] is left to the caller.
+template <int T, int K, bool OFFS = false>
+__device__ __forceinline__ void redc_chain_cols(uint32_t (&t)[NL], int64_t& acc, uint32_t (&m)[NL + 1],
                                                 const Fp* const (&x)[T], const int32_t (*const (&y)[T])[NL]) {
     if constexpr (K < 2 * NL - 1) {
         col_prods<T, K>(acc, x, y);
@@ -385,11 +390,11 @@ __device__ __forceinline__ void redc_chain_cols(uint32_t (&t)[NL], int64_t& acc,
             m[K] = ((uint32_t)acc * PINV28) & MASK28;
             mad_blk<true>(acc, (int32_t)m[K], (int32_t)PL[0]);
         } else {
-            col_reds<K, K - NL + 1, NL>(acc, m);
+            col_reds<K, K - NL + 1, NL + (OFFS ? 1 : 0)>(acc, m);
             t[K - NL] = (uint32_t)acc & MASK28;
         }
         acc >>= 28;  // arithmetic: for K < NL the low 28 bits are zero
-        redc_chain_cols<T, K + 1>(t, acc, m, x, y);
+        redc_chain_cols<T, K + 1, OFFS>(t, acc, m, x, y);
     }
 }
 
@@ -429,8 +434,8 @@ __device__ __forceinline__ void sq_col_prods(int64_t& acc, const int32_t (&d)[NL
 }
 // The single-chain Montgomery square scan. With SUB, h[K - NL] joins column
 // K of the high half (the fused X3 below); h[NL - 1] is left to the caller.
-template <int K, bool SUB>
-__device__ __forceinline__ void redc_sqr_chain_cols(uint32_t (&t)[NL], int64_t& acc, uint32_t (&m)[NL],
+template <int K, bool SUB, int NM>
+__device__ __forceinline__ void redc_sqr_chain_cols(uint32_t (&t)[NL], int64_t& acc, uint32_t (&m)[NM],
                                                     const int32_t (&d)[NL], const int32_t (&s)[NL],
                                                     const int32_t (&h)[NL]) {
     if constexpr (K < 2 * NL - 1) {
@@ -445,7 +450,7 @@ __device__ __forceinline__ void redc_sqr_chain_cols(uint32_t (&t)[NL], int64_t& 
             t[K - NL] = (uint32_t)acc & MASK28;
         }
         acc >>= 28;
-        redc_sqr_chain_cols<K + 1, SUB>(t, acc, m, d, s, h);
+        redc_sqr_chain_cols<K + 1, SUB, NM>(t, acc, m, d, s, h);
     }
 }
 #endif
@@ -457,17 +462,20 @@ __device__ __forceinline__ void redc_sqr_chain_cols(uint32_t (&t)[NL], int64_t& 
 // are below 2^28 in magnitude, so T = 4 terms (64 products per column) plus
 // 16 reduction products stay below 2^63. Returns the reduced value in
 // 28-bit limbs with its top limb signed (in the last word, sign carried by
-// the int64 returned as `top`).
-template <int T, bool CHAIN = false>
+// the int64 returned as `top`). With OFFS, e * p R joins the sum (e is 0 or
+// -1, per lane), so the reduced value drops by p where e = -1.
+template <int T, bool CHAIN = false, bool OFFS = false>
 __host__ __device__ __forceinline__ void redc_signed(uint32_t (&t)[NL], int64_t& top, const Fp* const (&x)[T],
-                                            const int32_t (*const (&y)[T])[NL]) {
-    uint32_t m[NL];
+                                            const int32_t (*const (&y)[T])[NL], int32_t e = 0) {
+    uint32_t m[NL + 1];
+    m[NL] = (uint32_t)e;
     int64_t acc = 0;
 #if defined(__HIP_DEVICE_COMPILE__)
     if constexpr (CHAIN) {
         // each column's limb products first, then its reduction products
-        redc_chain_cols<T, 0>(t, acc, m, x, y);
+        redc_chain_cols<T, 0, OFFS>(t, acc, m, x, y);
         top = acc;
+        if constexpr (OFFS) top += (int64_t)e * (int32_t)PL[NL - 1];
         return;
     }
 #endif
@@ -491,10 +499,20 @@ __host__ __device__ __forceinline__ void redc_signed(uint32_t (&t)[NL], int64_t&
             for (int u = 0; u < T; ++u) acc += (int64_t)(int32_t)x[u]->v[i] * (*y[u])[k - i];
             acc += (int64_t)((uint64_t)m[i] * PL[k - i]);
         }
+        if constexpr (OFFS) {
+#if defined(__HIP_DEVICE_COMPILE__)
+            // a mad into the column sum: the compiler would otherwise keep
+            // every e * p[i] in registers ahead of its column
+            mad_blk<true>(acc, e, (int32_t)PL[k - NL]);
+#else
+            acc += (int64_t)e * (int32_t)PL[k - NL];
+#endif
+        }
         t[k - NL] = (uint32_t)acc & MASK28;
         acc >>= 28;
     }
     top = acc;
+    if constexpr (OFFS) top += (int64_t)e * (int32_t)PL[NL - 1];
 }
 
 // Brings a reduced value in (-2p, 3p) (28-bit limbs 0..14 and a signed top
@@ -1039,8 +1057,9 @@ struct Fp2Ops {
 //   part 0: a0 b0 - a1 b1,   part 1: a1 b0 + a0 b1,
 // both as x y + u w with signed w (w = -b1 for part 0, +b1 for part 1), one
 // fused column sum and one Montgomery reduction per lane; the result is
-// brought into [0, 2p) by the two corrections of fp2_mul (part 0 lies in
-// (-p, 2p), part 1 in [0, 3p)). Every branch that depends on a value
+// brought into [0, 2p) by a conditional addition of 2p (fp2_lane_mul) or,
+// for raw operands and differences of products, by corrections in both
+// directions. Every branch that depends on a value
 // combines both lanes first, so control flow is uniform over each pair and
 // the DPP partner is always active.
 
@@ -1083,8 +1102,12 @@ __device__ __forceinline__ void lane_operands(int32_t (&y)[NL], int32_t (&w)[NL]
     }
 }
 
-// Own component of a b; ao is the partner's part of a.
-template <bool CHAIN = false>
+// Own component of a b; ao is the partner's part of a. For a and b in
+// [0, 2p) part 0 reduces to (-p, 2p) and part 1 to [0, 3p); part 1 takes
+// -p R into its sum, which puts it in [-p, 2p) as well, so one conditional
+// addition of 2p brings either to [0, 2p). RAW (operands of magnitude below
+// 2p): both parts lie in (-2p, 3p), corrected in both directions.
+template <bool CHAIN = false, bool RAW = false>
 __device__ __forceinline__ void fp2_lane_mul(Fp& r, const Fp& a, const Fp& ao, const Fp& b) {
     int32_t y[NL], w[NL];
     lane_operands(y, w, b, false);
@@ -1092,9 +1115,13 @@ __device__ __forceinline__ void fp2_lane_mul(Fp& r, const Fp& a, const Fp& ao, c
     const int32_t(*const yy[2])[NL] = {&y, &w};
     uint32_t t[NL];
     int64_t top;
-    redc_signed<2, CHAIN>(t, top, x, yy);
-    // part 0 lies in (-p, 2p), part 1 in [0, 3p)
-    redc_fold(r, t, top, true);
+    if constexpr (RAW) {
+        redc_signed<2, CHAIN>(t, top, x, yy);
+        redc_fold(r, t, top, true);
+    } else {
+        redc_signed<2, CHAIN, true>(t, top, x, yy, lane_part() != 0 ? -1 : 0);
+        redc_fold(r, t, top, false);
+    }
 }
 
 // Own component of a b - c d with one reduction. With every operand in
@@ -1179,9 +1206,9 @@ struct Fp2LaneOpsT {
     }
     // Raw differences, as FpOps has them: a difference that only feeds
     // products stays limb by limb (signed limbs, value of magnitude below
-    // 2p), and the lane products take signed operands anyway. mul_raw is the
-    // general lane product (raw operands keep each part below 8p^2);
-    // mul_sub_raw (raw a, b) and the fused X3 fold wider ranges.
+    // 2p), and the lane products take signed operands anyway. mul_raw takes
+    // raw operands (each part's sum below 8p^2 in magnitude, two
+    // corrections); mul_sub_raw (raw a, b) and the fused X3 fold wider ranges.
     __device__ static __forceinline__ void sub_raw(F& r, const F& a, const F& b) { fp_sub_raw(r, a, b); }
     __device__ static __forceinline__ void sqr_raw(F& r, const F& a) {
         uint32_t t[NL];
@@ -1189,7 +1216,11 @@ struct Fp2LaneOpsT {
         fp2_lane_sqr_cols<CHAIN>(t, top, a);
         redc_fold(r, t, top, true);
     }
-    __device__ static __forceinline__ void mul_raw(F& r, const F& a, const F& b) { mul(r, a, b); }
+    __device__ static __forceinline__ void mul_raw(F& r, const F& a, const F& b) {
+        Fp ao;
+        fp_pair_swap(ao, a);
+        fp2_lane_mul<CHAIN, true>(r, a, ao, b);
+    }
     __device__ static __forceinline__ void mul_sub_raw(F& r, const F& a, const F& b, const F& c, const F& d) {
         Fp ao, co;
         fp_pair_swap(ao, a);
