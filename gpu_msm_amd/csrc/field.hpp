@@ -638,9 +638,9 @@ constexpr uint32_t P8L[NL] = {0x5855558, 0xe001355, 0xdae6188, 0x29462b9, 0xf72b
                               0x412736e, 0xd638b01, 0xbad325d, 0x17f1907, 0xd25bc50, 0xb20004a,
                               0xa4b2d56, 0xa9007e1, 0x9462f29, 0x1e6f707d};
 
-// Brings a reduced value in (-4p, 5p) (28-bit limbs 0..14 and a signed top
-// limb) to [0, 2p): add 4p when negative (now in [0, 5p)), then subtract 4p
-// and 2p where that does not borrow.
+// Brings a reduced value in (-4p, 4p) (28-bit limbs 0..14 and a signed top
+// limb) to [0, 2p): add 4p when negative (now in [0, 4p)), then subtract 2p
+// where that does not borrow.
 __device__ __forceinline__ void redc_fold_wide(Fp& r, uint32_t (&t)[NL], int64_t top) {
     const uint32_t msk = top < 0 ? 0xFFFFFFFFu : 0u;
     int32_t c = 0;
@@ -651,7 +651,6 @@ __device__ __forceinline__ void redc_fold_wide(Fp& r, uint32_t (&t)[NL], int64_t
         c = v >> 28;
     }
     t[NL - 1] = (uint32_t)((int32_t)top + (int32_t)(P4L[NL - 1] & msk) + c);
-    fp_cond_sub(t, P4L);
     fp_cond_sub(t, P2L);
 #pragma unroll
     for (int i = 0; i < NL; ++i) r.v[i] = t[i];
@@ -1126,8 +1125,11 @@ __device__ __forceinline__ void fp2_lane_mul(Fp& r, const Fp& a, const Fp& ao, c
 
 // Own component of a b - c d with one reduction. With every operand in
 // [0, 2p) both parts lie in (-8p^2, 8p^2) before and in (-2p, 3p) after the
-// reduction; with a and b raw (magnitude below 2p) in (-16p^2, 16p^2) and
-// (-4p, 5p), which `wide` folds.
+// reduction. `wide`: a and b raw (magnitude below 2p), c and d in [0, 2p);
+// part 0 (a0 b0 - a1 b1 - c0 d0 + c1 d1) lies in (-12p^2, 12p^2) and part 1
+// (a1 b0 + a0 b1 - c1 d0 - c0 d1) in (-16p^2, 8p^2), so with p < 0.238 R
+// the reduced values lie in (-2.86p, 3.86p) and (-3.81p, 2.91p), within the
+// (-4p, 4p) that redc_fold_wide takes.
 template <bool CHAIN = false>
 __device__ __forceinline__ void fp2_lane_mul_sub(Fp& r, const Fp& a, const Fp& ao, const Fp& b, const Fp& c,
                                                  const Fp& co, const Fp& d, bool wide = false) {
