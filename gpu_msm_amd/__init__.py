@@ -62,7 +62,7 @@ EXPORTS = [
     "gm_set_profiling", "gm_profile_last", "gm_last_msm_info",
     "gm_point_encoded_size", "gm_points_serialize", "gm_points_deserialize", "gm_points_check",
     "gm_points_to_canonical", "gm_points_from_canonical", "gm_generator_multiple",
-    "gm_point_msm_noalloc", "gm_lane_ops_check",
+    "gm_point_msm_noalloc", "gm_field_ops_check",
 ]
 
 
@@ -137,7 +137,7 @@ def lib() -> ctypes.CDLL:
         "gm_points_to_canonical": ([i, vp, vp, i, u32], i),
         "gm_points_from_canonical": ([i, vp, vp, i, u32], i),
         "gm_generator_multiple": ([i, vp, i, vp], i),
-        "gm_lane_ops_check": ([i, i, vp, vp, vp, vp, vp, i, vp], i),
+        "gm_field_ops_check": ([i, i, i, vp, vp, vp, vp, vp, i, vp], i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

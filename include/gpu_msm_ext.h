@@ -249,18 +249,19 @@ int gm_points_from_canonical(int curve, void* out, const void* points, int n, ui
 
 /* ------------------------------------------------------------ diagnostics */
 
-/* No counterpart in the reference; for tests of the G2 bucket kernels'
- * lazily reduced Fp2 products. Runs one product per element on the lane
- * pairs those kernels use (chain != 0: the accumulation's single-chain
- * column sums, else the fix-up and tail kernels' form), on device arrays of
- * n elements of 2 x 16 uint32 28-bit limbs (Montgomery form, c0 then c1):
- * op 0 a b, 1 a b (a, b raw), 2 a b - c d, 3 a b - c d (a, b raw),
+/* No counterpart in the reference; for tests of the bucket kernels' lazily
+ * reduced field products. Runs one product per element on device arrays of n
+ * elements of 16 uint32 28-bit limbs (G1, the Fp products, one element per
+ * lane) or 2 x 16 (G2, c0 then c1: the Fp2 products on the lane pairs those
+ * kernels use, one element per pair), Montgomery form; chain != 0 selects the
+ * accumulation's single-chain column sums, else the fix-up and tail kernels'
+ * form. op 0 a b, 1 a b (a, b raw), 2 a b - c d, 3 a b - c d (a, b raw),
  * 4 a^2 (a raw), 5 a^2 - b - 2c (a raw). Operands lie in [0, 2p); raw ones
  * are limb-wise differences of two such values. Results lie in [0, 2p) with
  * normalised limbs. Asynchronous on `stream` (a hipStream_t); returns a
  * hipError_t code. */
-int gm_lane_ops_check(int chain, int op, const void* a, const void* b, const void* c, const void* d, void* out,
-                      int n, void* stream);
+int gm_field_ops_check(int curve, int chain, int op, const void* a, const void* b, const void* c, const void* d,
+                       void* out, int n, void* stream);
 
 #ifdef __cplusplus
 }
