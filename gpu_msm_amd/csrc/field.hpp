@@ -515,11 +515,10 @@ __host__ __device__ __forceinline__ void redc_signed(uint32_t (&t)[NL], int64_t&
     if constexpr (OFFS) top += (int64_t)e * (int32_t)PL[NL - 1];
 }
 
-// Brings a reduced value in (-2p, 3p) (28-bit limbs 0..14 and a signed top
-// limb) to [0, 2p): add 2p when negative, then subtract 2p unless that
-// borrows. `sub_pass` = false skips the second step for values known to be
-// below 2p.
-__host__ __device__ __forceinline__ void redc_fold(Fp& r, uint32_t (&t)[NL], int64_t top, bool sub_pass) {
+// Brings a reduced value in (-2p, 2p) (28-bit limbs 0..14 and a signed top
+// limb) to [0, 2p): add 2p when negative. Values in (-2p, 3p) take
+// fold_est<-1, 1> below.
+__host__ __device__ __forceinline__ void redc_fold(Fp& r, uint32_t (&t)[NL], int64_t top) {
     const uint32_t msk = top < 0 ? 0xFFFFFFFFu : 0u;
     int32_t c = 0;
 #pragma unroll
@@ -529,22 +528,8 @@ __host__ __device__ __forceinline__ void redc_fold(Fp& r, uint32_t (&t)[NL], int
         c = v >> 28;
     }
     t[NL - 1] = (uint32_t)((int32_t)top + (int32_t)(P2L[NL - 1] & msk) + c);
-    if (!sub_pass) {
 #pragma unroll
-        for (int i = 0; i < NL; ++i) r.v[i] = t[i];
-        return;
-    }
-    uint32_t d[NL];
-    int32_t br = 0;
-#pragma unroll
-    for (int i = 0; i < NL; ++i) {
-        const int32_t v = (int32_t)t[i] - (int32_t)P2L[i] + br;
-        d[i] = (uint32_t)v & MASK28;
-        br = v >> 28;
-    }
-    const bool keep = br < 0;
-#pragma unroll
-    for (int i = 0; i < NL; ++i) r.v[i] = keep ? t[i] : d[i];
+    for (int i = 0; i < NL; ++i) r.v[i] = t[i];
 }
 
 // r = (a b - c d) R^{-1} mod p with one reduction. Inputs in [0, 2p): the
@@ -562,7 +547,7 @@ __host__ __device__ __forceinline__ void fp_mul_sub(Fp& r, const Fp& a, const Fp
     uint32_t t[NL];
     int64_t top;
     redc_signed<2>(t, top, x, y);
-    redc_fold(r, t, top, false);
+    redc_fold(r, t, top);
 }
 
 // Raw differences. A value used only as a product operand can skip the
@@ -612,14 +597,9 @@ __device__ __forceinline__ void fp_sqr_raw(Fp& r, const Fp& a) {
     r.v[NL - 1] = (uint32_t)acc;  // the value is in [0, 2p): the top limb is non-negative
 }
 
-// 4p in 28-bit limbs (4p < R, so 16 limbs hold it)
-constexpr uint32_t P4L[NL] = {0xac2aaac, 0x70009aa, 0xed730c4, 0x14a315c, 0x7b95bb9, 0x0f88eb3,
-                              0xa0939b7, 0xeb1c580, 0xdd6992e, 0x0bf8c83, 0x692de28, 0x5900025,
-                              0xd2596ab, 0xd4803f0, 0xca31794, 0xf37b83e};
-
 // Subtracts m (4p or 2p limbs) from the normalised value t when that does not
 // borrow.
-__device__ __forceinline__ void fp_cond_sub(uint32_t (&t)[NL], const uint32_t (&m)[NL]) {
+__host__ __device__ __forceinline__ void fp_cond_sub(uint32_t (&t)[NL], const uint32_t (&m)[NL]) {
     uint32_t d[NL];
     int32_t br = 0;
 #pragma unroll
@@ -633,36 +613,72 @@ __device__ __forceinline__ void fp_cond_sub(uint32_t (&t)[NL], const uint32_t (&
     for (int i = 0; i < NL; ++i) t[i] = keep ? t[i] : d[i];
 }
 
-// 8p in 28-bit limbs, the top limb holding 29 bits (8p > R)
-constexpr uint32_t P8L[NL] = {0x5855558, 0xe001355, 0xdae6188, 0x29462b9, 0xf72b772, 0x1f11d66,
-                              0x412736e, 0xd638b01, 0xbad325d, 0x17f1907, 0xd25bc50, 0xb20004a,
-                              0xa4b2d56, 0xa9007e1, 0x9462f29, 0x1e6f707d};
+// floor(j 2p / 2^420) for j >= 0: limb 15 of j 2p with the carries of its
+// lower limbs.
+constexpr int32_t mul2p_top(int j) {
+    int64_t c = 0;
+    for (int i = 0; i < NL - 1; ++i) c = ((int64_t)j * P2L[i] + c) >> 28;
+    return (int32_t)((int64_t)j * P2L[NL - 1] + c);
+}
+// floor(j 2p / 2^420) for any small j (for j != 0 the low 420 bits of j 2p
+// are not zero: the lowest limb of 2p is not).
+constexpr int32_t mul2p_floor_top(int j) { return j >= 0 ? mul2p_top(j) : -mul2p_top(-j) - 1; }
 
-// Brings a reduced value in (-4p, 4p) (28-bit limbs 0..14 and a signed top
-// limb) to [0, 2p): add 4p when negative (now in [0, 4p)), then subtract 2p
-// where that does not borrow.
-__device__ __forceinline__ void redc_fold_wide(Fp& r, uint32_t (&t)[NL], int64_t top) {
-    const uint32_t msk = top < 0 ? 0xFFFFFFFFu : 0u;
+// Brings a value v in [KMIN 2p, (KMAX + 1) 2p) to [0, 2p) in one pass: v has
+// normalised 28-bit limbs 0..14 and a signed limb 15 `top` (bits 420 and up).
+// The quotient k = floor(v / 2p) is the number of multiples j 2p
+// (KMIN < j <= KMAX) at or below v, less -KMIN, and each comparison needs only
+// top: v >= j 2p when top > floor(j 2p / 2^420), v < j 2p when top is smaller
+// (for j = 0 the sign of top decides). Equal top limbs leave the comparison
+// open; counting it as below makes k at most one short. v - k 2p (limbs
+// t_i - k 2p_i with carries) then lies in [0, 2p), or in [0, 4p) after a tie,
+// for which one conditional subtraction of 2p follows. A tie needs the top
+// 26 bits of v to equal those of a multiple of 2p: about once in 10^8 values,
+// so the subtraction is a branch the whole wave skips almost always.
+template <int KMIN, int KMAX>
+__host__ __device__ __forceinline__ void fold_est(Fp& r, uint32_t (&t)[NL], int32_t top) {
+    int32_t k = KMIN;
+    bool tie = false;
+#pragma unroll
+    for (int j = KMIN + 1; j <= KMAX; ++j) {
+        if (j == 0) {
+            k += top >= 0 ? 1 : 0;
+        } else {
+            const int32_t tj = mul2p_floor_top(j);
+            k += top > tj ? 1 : 0;
+            tie = tie || top == tj;
+        }
+    }
     int32_t c = 0;
 #pragma unroll
     for (int i = 0; i < NL - 1; ++i) {
-        const int32_t v = (int32_t)t[i] + (int32_t)(P4L[i] & msk) + c;
+        const int32_t v = (int32_t)t[i] - k * (int32_t)P2L[i] + c;
         t[i] = (uint32_t)v & MASK28;
         c = v >> 28;
     }
-    t[NL - 1] = (uint32_t)((int32_t)top + (int32_t)(P4L[NL - 1] & msk) + c);
-    fp_cond_sub(t, P2L);
+    t[NL - 1] = (uint32_t)(top - k * (int32_t)P2L[NL - 1] + c);
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (__builtin_amdgcn_ballot_w64(tie) != 0) fp_cond_sub(t, P2L);
+#else
+    if (tie) fp_cond_sub(t, P2L);
+#endif
 #pragma unroll
     for (int i = 0; i < NL; ++i) r.v[i] = t[i];
+}
+
+// Brings a reduced value in (-4p, 4p) (28-bit limbs 0..14 and a signed top
+// limb) to [0, 2p), in one pass (fold_est).
+__device__ __forceinline__ void redc_fold_wide(Fp& r, uint32_t (&t)[NL], int64_t top) {
+    fold_est<-2, 1>(r, t, (int32_t)top);
 }
 
 // r = a^2 R^{-1} - b - 2c for a raw a and b, c in [0, 2p), in [0, 2p) (the
 // X3 of every XYZZ formula). b + 2c - 6p is subtracted from the high half of
 // the square's column sums, i.e. (b + 2c - 6p) R before the reduction: the
 // square reduces to [0, 1.95p) and b + 2c < 6p, so the result lies in
-// (0, 8p) with normalised limbs, and two conditional subtractions (4p, then
-// 2p) bring it to [0, 2p). Limb i of 6p - b - 2c is 6 p_i - b_i - 2 c_i, in
-// (-3 * 2^28, 2^31).
+// (0, 8p) with normalised limbs, and one quotient-estimate pass
+// (fold_est<0, 3>) brings it to [0, 2p). Limb i of 6p - b - 2c is
+// 6 p_i - b_i - 2 c_i, in (-3 * 2^28, 2^31).
 __device__ __forceinline__ void fp_sqr_raw_sub2(Fp& r, const Fp& a, const Fp& b, const Fp& c) {
     uint32_t m[NL], t[NL];
     int32_t d[NL], s[NL];
@@ -695,11 +711,7 @@ __device__ __forceinline__ void fp_sqr_raw_sub2(Fp& r, const Fp& a, const Fp& b,
         acc >>= 28;
     }
     acc += (int64_t)((int32_t)(6u * PL[NL - 1]) - (int32_t)b.v[NL - 1] - 2 * (int32_t)c.v[NL - 1]);
-    t[NL - 1] = (uint32_t)acc;  // in (0, 8p): non-negative, below 2^29
-    fp_cond_sub(t, P4L);
-    fp_cond_sub(t, P2L);
-#pragma unroll
-    for (int i = 0; i < NL; ++i) r.v[i] = t[i];
+    fold_est<0, 3>(r, t, (int32_t)acc);  // in (0, 8p): non-negative, below 2^29
 }
 
 // r = a b R^{-1} for a in [0, 2p) and a raw b: the product lies in
@@ -711,7 +723,7 @@ __device__ __forceinline__ void fp_mul_raw(Fp& r, const Fp& a, const Fp& b) {
     uint32_t t[NL];
     int64_t top;
     redc_signed<1>(t, top, x, y);
-    redc_fold(r, t, top, false);
+    redc_fold(r, t, top);
 }
 
 // r = (a b - c d) R^{-1} for raw a, b and c, d in [0, 2p): a b lies in
@@ -727,7 +739,7 @@ __device__ __forceinline__ void fp_mul_sub_raw(Fp& r, const Fp& a, const Fp& b, 
     uint32_t t[NL];
     int64_t top;
     redc_signed<2>(t, top, x, y);
-    redc_fold(r, t, top, false);
+    redc_fold(r, t, top);
 }
 
 // ------------------------------------------------------------------ Fp2
@@ -925,7 +937,7 @@ __device__ __forceinline__ void fp_mul_raw_chain(Fp& r, const Fp& a, const Fp& b
     uint32_t t[NL];
     int64_t top;
     redc_signed<1, true>(t, top, x, y);
-    redc_fold(r, t, top, false);
+    redc_fold(r, t, top);
 #else
     fp_mul_raw(r, a, b);
 #endif
@@ -940,7 +952,7 @@ __device__ __forceinline__ void fp_mul_sub_raw_chain(Fp& r, const Fp& a, const F
     uint32_t t[NL];
     int64_t top;
     redc_signed<2, true>(t, top, x, y);
-    redc_fold(r, t, top, false);
+    redc_fold(r, t, top);
 #else
     fp_mul_sub_raw(r, a, b, c, d);
 #endif
@@ -963,8 +975,7 @@ __device__ __forceinline__ void fp_sqr_raw_chain(Fp& r, const Fp& a) {
     fp_sqr_raw(r, a);
 #endif
 }
-// a^2 - b - 2c for a raw a (fp_sqr_raw_sub2: (0, 8p) before the two
-// conditional subtractions).
+// a^2 - b - 2c for a raw a (fp_sqr_raw_sub2: (0, 8p) before the fold).
 __device__ __forceinline__ void fp_sqr_raw_sub2_chain(Fp& r, const Fp& a, const Fp& b, const Fp& c) {
 #if defined(__HIP_DEVICE_COMPILE__)
     uint32_t m[NL], t[NL];
@@ -978,11 +989,7 @@ __device__ __forceinline__ void fp_sqr_raw_sub2_chain(Fp& r, const Fp& a, const 
     int64_t acc = 0;
     redc_sqr_chain_cols<0, true>(t, acc, m, d, s, h);
     acc += (int64_t)h[NL - 1];
-    t[NL - 1] = (uint32_t)acc;  // in (0, 8p): non-negative, below 2^29
-    fp_cond_sub(t, P4L);
-    fp_cond_sub(t, P2L);
-#pragma unroll
-    for (int i = 0; i < NL; ++i) r.v[i] = t[i];
+    fold_est<0, 3>(r, t, (int32_t)acc);  // in (0, 8p): non-negative, below 2^29
 #else
     fp_sqr_raw_sub2(r, a, b, c);
 #endif
@@ -1116,10 +1123,10 @@ __device__ __forceinline__ void fp2_lane_mul(Fp& r, const Fp& a, const Fp& ao, c
     int64_t top;
     if constexpr (RAW) {
         redc_signed<2, CHAIN>(t, top, x, yy);
-        redc_fold(r, t, top, true);
+        fold_est<-1, 1>(r, t, (int32_t)top);
     } else {
         redc_signed<2, CHAIN, true>(t, top, x, yy, lane_part() != 0 ? -1 : 0);
-        redc_fold(r, t, top, false);
+        redc_fold(r, t, top);
     }
 }
 
@@ -1144,7 +1151,7 @@ __device__ __forceinline__ void fp2_lane_mul_sub(Fp& r, const Fp& a, const Fp& a
     if (wide)
         redc_fold_wide(r, t, top);
     else
-        redc_fold(r, t, top, true);
+        fold_est<-1, 1>(r, t, (int32_t)top);
 }
 
 // Own component of a^2 for a raw a (limbs of magnitude below 2^28, value of
@@ -1216,7 +1223,7 @@ struct Fp2LaneOpsT {
         uint32_t t[NL];
         int64_t top;
         fp2_lane_sqr_cols<CHAIN>(t, top, a);
-        redc_fold(r, t, top, true);
+        fold_est<-1, 1>(r, t, (int32_t)top);
     }
     __device__ static __forceinline__ void mul_raw(F& r, const F& a, const F& b) {
         Fp ao;
@@ -1232,8 +1239,8 @@ struct Fp2LaneOpsT {
     // Own component of a^2 - b - 2c (a raw, b and c in [0, 2p)): the square
     // reduces to (-2p, 3p); 8p - b - 2c (limbs 8 p_i - b_i - 2 c_i, in
     // (-3 * 2^28, 2^31)) is added with the carries, so the sum lies in
-    // (0, 11p), and subtracting 8p, 4p and 2p where that does not borrow
-    // brings it to [0, 2p).
+    // (0, 11p), and one quotient-estimate pass (fold_est<0, 5>) brings it to
+    // [0, 2p).
     __device__ static __forceinline__ void sqr_sub2(F& r, const F& a, const F& b, const F& c) {
         uint32_t t[NL];
         int64_t top;
@@ -1246,12 +1253,7 @@ struct Fp2LaneOpsT {
             acc >>= 28;
         }
         acc += top + ((int64_t)(8u * PL[NL - 1]) - (int32_t)b.v[NL - 1] - 2 * (int64_t)(int32_t)c.v[NL - 1]);
-        t[NL - 1] = (uint32_t)acc;  // in (0, 11p): non-negative, below 2^31
-        fp_cond_sub(t, P8L);
-        fp_cond_sub(t, P4L);
-        fp_cond_sub(t, P2L);
-#pragma unroll
-        for (int i = 0; i < NL; ++i) r.v[i] = t[i];
+        fold_est<0, 5>(r, t, (int32_t)acc);  // in (0, 11p): non-negative, below 2^31
     }
     __device__ static __forceinline__ void sqr_sub2z(F& r, const F& a, const F& c) {
         Fp z;

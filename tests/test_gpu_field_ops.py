@@ -6,17 +6,21 @@ relies on.
 
 These products reduce lazily: a sum of limb products (signed where an operand
 is a raw difference) is reduced once, then brought to [0, 2p) by as few
-conditional corrections as its value range allows. G1 (one Fp element per
-lane): a product of values below 2p needs none, a raw product or difference
-of products one, the fused X3 two. G2 (one Fp2 element per lane pair, each
-lane reducing its own component): a plain product takes -p R into part 1's
-sum so that one correction suffices, a difference of products with raw
-factors folds (-4p, 4p) in two, the fused X3 (0, 11p) in three. Whether those
-ranges hold is decided by operands near their bounds (components near 2p, raw
+corrections as its value range allows. G1 (one Fp element per lane): a
+product of values below 2p needs none, a raw product or difference of
+products one conditional addition, the fused X3 one pass. G2 (one Fp2 element
+per lane pair, each lane reducing its own component): a plain product takes
+-p R into part 1's sum so that one conditional addition suffices; raw
+products, differences of products and the fused X3 take one pass each. Those
+passes subtract k 2p with the quotient k read off the top limb, which can tie
+with a multiple of 2p and then takes a correcting branch. Whether the ranges
+hold is decided by operands near their bounds (components near 2p, raw
 differences near +-2p), which random field elements almost never are; the
 operands here are drawn mostly from such edge values, after the worst-case
-sums of each product. Both forms run: the accumulation's single-chain column
-sums (chain = 1) and the fix-up and tail kernels' (chain = 0)."""
+sums of each product, and for the fused X3 also values exactly at, just below
+and just above multiples of 2p. Both forms run: the accumulation's
+single-chain column sums (chain = 1) and the fix-up and tail kernels'
+(chain = 0)."""
 import numpy as np
 import pytest
 
@@ -100,7 +104,10 @@ def extremes(curve: int, op: int):
             return [((t,), (t,), (t,), (t,)), ((0,), (t,), (t,), (t,)), ((t,), (t,), z, z)]
         if op in (1, 3):  # a, b raw; c, d in [0, 2p)
             return [((t,), (-t,), (t,), (t,)), ((-t,), (-t,), z, z), ((-t,), (t,), (0,), (t,))]
-        return [((t,), (t,), (t,), z), ((-t,), z, z, z), ((0,), (t,), (t,), z), ((t,), z, z, z)]
+        base = [((t,), (t,), (t,), z), ((-t,), z, z, z), ((0,), (t,), (t,), z), ((t,), z, z, z)]
+        if op == 5:  # a = 0: 6p - b - 2c exactly at, just below and just above the multiples of 2p
+            base += [(z, (b,), (c,), z) for b, c in ((2, t), (3, t), (1, t), (0, P), (1, P), (0, 0), (1, 0))]
+        return base
     z = (0, 0)
     if op in (0, 2):
         return [((t, t), (t, t), (t, t), (t, t)), ((0, t), (0, t), (t, 0), (t, 0)),
@@ -108,8 +115,12 @@ def extremes(curve: int, op: int):
     if op in (1, 3):
         return [((t, -t), (t, -t), (t, t), (t, t)), ((t, -t), (t, t), (0, t), (0, t)),
                 ((-t, -t), (-t, -t), (t, 0), (t, 0)), ((-t, t), (t, -t), z, z)]
-    return [((t, -t), (t, t), (t, t), z), ((-t, -t), z, z, z), ((t, t), (t, t), (t, t), z),
+    base = [((t, -t), (t, t), (t, t), z), ((-t, -t), z, z, z), ((t, t), (t, t), (t, t), z),
             ((-t, t), (t, 0), (0, t), z)]
+    if op == 5:  # a = 0: 8p - b - 2c exactly at, just below and just above the multiples of 2p
+        base += [(z, (2, 3), (t, t), z), (z, (1, 0), (t, 0), z), (z, (0, 1), (P, P), z),
+                 (z, (0, 0), (0, 0), z), (z, (1, 2), (0, t), z)]
+    return base
 
 
 def build_case(curve: int, op: int, seed: int):
