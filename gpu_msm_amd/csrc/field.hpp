@@ -169,6 +169,18 @@ __device__ __forceinline__ void fp_neg(Fp& r, const Fp& a) {
     fp_sub(r, z, a);
 }
 
+// r = p - a for a canonical a in [0, p): in (0, p] (p, the other lazy form
+// of 0, for a = 0), with normalised limbs, in one borrow chain.
+__device__ __forceinline__ void fp_neg_canon(Fp& r, const Fp& a) {
+    int32_t br = 0;
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+        const int32_t v = (int32_t)PL[i] - (int32_t)a.v[i] + br;
+        r.v[i] = (uint32_t)v & MASK28;
+        br = v >> 28;
+    }
+}
+
 // Canonical representative in [0, p).
 __device__ __forceinline__ void fp_canon(Fp& r, const Fp& a) {
     uint32_t d[NL];
@@ -880,6 +892,8 @@ struct FpOps {
     __device__ static __forceinline__ void sub(F& r, const F& a, const F& b) { fp_sub(r, a, b); }
     __device__ static __forceinline__ void dbl(F& r, const F& a) { fp_dbl(r, a); }
     __device__ static __forceinline__ void neg(F& r, const F& a) { fp_neg(r, a); }
+    // -a for a canonical a (the resident bases' coordinates)
+    __device__ static __forceinline__ void neg_canon(F& r, const F& a) { fp_neg_canon(r, a); }
     __device__ static __forceinline__ void mul(F& r, const F& a, const F& b) { fp_mul(r, a, b); }
     __device__ static __forceinline__ void sqr(F& r, const F& a) { fp_sqr(r, a); }
     // r = a b - c d
@@ -1187,6 +1201,8 @@ struct Fp2LaneOpsT {
     __device__ static __forceinline__ void sub(F& r, const F& a, const F& b) { fp_sub(r, a, b); }
     __device__ static __forceinline__ void dbl(F& r, const F& a) { fp_dbl(r, a); }
     __device__ static __forceinline__ void neg(F& r, const F& a) { fp_neg(r, a); }
+    // own component of -a for a canonical a (the resident bases' coordinates)
+    __device__ static __forceinline__ void neg_canon(F& r, const F& a) { fp_neg_canon(r, a); }
     __device__ static __forceinline__ void mul(F& r, const F& a, const F& b) {
         Fp ao;
         fp_pair_swap(ao, a);

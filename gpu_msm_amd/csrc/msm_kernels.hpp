@@ -72,6 +72,9 @@ __device__ __forceinline__ void fp_to_u64x7(uint64_t* w, const Fp& a) {
     }
 }
 
+// Every element is stored canonical, in [0, p): the accumulation negates a
+// base's y as p - y. Montgomery-form input (below 2^448 < 5p) is reduced by up
+// to four subtractions of p.
 __device__ __forceinline__ void import_elem(Fp& r, const uint64_t* w, bool to_mont) {
     fp_from_u64x7(r, w);
     if (to_mont) {
@@ -79,6 +82,9 @@ __device__ __forceinline__ void import_elem(Fp& r, const uint64_t* w, bool to_mo
         fp_set_const(r2, R2L);
         fp_mul(r, r, r2);
         fp_canon(r, r);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) fp_cond_sub(r.v, PL);
     }
 }
 __device__ __forceinline__ void import_elem(Fp2& r, const uint64_t* w, bool to_mont) {
@@ -354,7 +360,7 @@ __global__ void __launch_bounds__(64, 2) k_accumulate(const MsmDesc* __restrict_
             first = false;
         }
         if (aff_is_inf(pt)) continue;
-        if ((uint32_t)e >> 31) O::neg(pt.y, pt.y);
+        if ((uint32_t)e >> 31) O::neg_canon(pt.y, pt.y);  // resident bases are canonical
         xyzz_add_aff<O>(acc, pt.x, pt.y);
     }
     if (first && off[cur] < j0)
