@@ -509,3 +509,34 @@ def test_global_sort_path_wide_windows(curve):
     for k, (m_, bo, so, _) in enumerate(specs):
         w = o.msm(curve, bases[bo:bo + m_], ss[so:so + m_])
         assert same(got[k:k + 1], w), k
+
+
+@pytest.mark.parametrize("curve", ["g1", "g2"])
+def test_montgomery_upload_stores_canonical_coordinates(curve):
+    """Bases uploaded in Montgomery form are stored canonical, below p (the
+    accumulation negates a base's y as p - y, which needs it): coordinates
+    given as x + k p (k = 1..3, kept below 2^448) download as x and give the
+    same MSM, bit for bit, as the canonical upload."""
+    cv = gm.CURVE_G1 if curve == "g1" else gm.CURVE_G2
+    ks = o.gen_u64(77, 256)
+    ref = gm.Bases.from_k(cv, ks)
+    recs = ref.download(0, 256)
+    big = recs.copy()
+    rng = np.random.default_rng(3)
+    lifted = 0
+    for i in range(len(big)):
+        for f in ("x", "y"):
+            w = big[f][i]
+            for e in range(len(w) // 7):
+                v = sum(int(w[7 * e + j]) << (64 * j) for j in range(7))
+                v2 = v + int(rng.integers(1, 4)) * m.P
+                if v2 < 1 << 448:
+                    w[7 * e:7 * e + 7] = [(v2 >> (64 * j)) & ((1 << 64) - 1) for j in range(7)]
+                    lifted += 1
+    assert lifted > 256
+    up = gm.Bases.upload(cv, big, montgomery=True)
+    assert same(up.download(0, 256), recs)
+    s = gm.Scalars.upload(o.gen_scalars_modr(78, 256))
+    assert same(gm.msm_resident(up, s, scalar_bits=299), gm.msm_resident(ref, s, scalar_bits=299))
+    up.free()
+    ref.free()
