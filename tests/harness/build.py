@@ -1,6 +1,7 @@
 """Compiles the caller harnesses against the built library:
 rust_flow (C11 with gcc, on include/rust_wrapper.h), device_api and msm_flow
-(C++17 with g++ on include/curve.h and the HIP runtime API headers). Binaries go to
+(C++17 with g++ on include/curve.h and the HIP runtime API headers); and
+fold_host, a host build of the field code's one-pass fold (hipcc, no library). Binaries go to
 tests/harness/_build/ and find libgpu_msm_amd.so through an $ORIGIN rpath, so
 the tree can move.
 
@@ -28,7 +29,11 @@ COMMANDS = {
                  "-I", INC, "-I", f"{ROCM}/include", os.path.join(HERE, "msm_flow.cpp"), "-o",
                  os.path.join(OUT, "msm_flow"), *LINK, "-L", f"{ROCM}/lib", "-lamdhip64"],
 }
-SOURCES = {"rust_flow": "rust_flow.c", "device_api": "device_api.cpp", "msm_flow": "msm_flow.cpp"}
+COMMANDS["fold_host"] = [os.path.join(ROCM, "bin", "hipcc"), "-std=c++17", "-O1", "--offload-arch=gfx950",
+                         "-I", os.path.join(ROOT, "gpu_msm_amd", "csrc"), "-I", INC,
+                         os.path.join(HERE, "fold_host.cpp"), "-o", os.path.join(OUT, "fold_host")]
+SOURCES = {"rust_flow": "rust_flow.c", "device_api": "device_api.cpp", "msm_flow": "msm_flow.cpp",
+           "fold_host": "fold_host.cpp"}
 
 
 def build(name: str) -> str:
@@ -37,6 +42,8 @@ def build(name: str) -> str:
     exe = os.path.join(OUT, name)
     deps = [os.path.join(HERE, SOURCES[name]), os.path.join(LIBDIR, "libgpu_msm_amd.so")]
     deps += [os.path.join(INC, h) for h in os.listdir(INC)]
+    if name == "fold_host":  # the field code itself, not the library
+        deps = [os.path.join(HERE, SOURCES[name]), os.path.join(ROOT, "gpu_msm_amd", "csrc", "field.hpp")]
     if os.path.exists(exe) and os.path.getmtime(exe) >= max(os.path.getmtime(d) for d in deps):
         return exe
     os.makedirs(OUT, exist_ok=True)
