@@ -678,6 +678,39 @@ __host__ __device__ __forceinline__ void fold_est(Fp& r, uint32_t (&t)[NL], int3
     for (int i = 0; i < NL; ++i) r.v[i] = t[i];
 }
 
+// r = a - b - 2c for a, b, c in [0, 2p), in [0, 2p): a - b - 2c + 6p lies in
+// (0, 8p), summed limb by limb with the carries (limb i of 6p - b - 2c is in
+// (-3 * 2^28, 6 * 2^28)), then one fold pass. The X3 = RR - PPP - 2Q and
+// X3 = M^2 - 2S of the split point formulas, in two passes instead of the
+// four or six of separate subtractions.
+__device__ __forceinline__ void fp_sub_b2c(Fp& r, const Fp& a, const Fp& b, const Fp& c) {
+    uint32_t t[NL];
+    int32_t acc = 0;
+#pragma unroll
+    for (int i = 0; i < NL - 1; ++i) {
+        acc += (int32_t)a.v[i] - (int32_t)b.v[i] - 2 * (int32_t)c.v[i] + (int32_t)(6u * PL[i]);
+        t[i] = (uint32_t)acc & MASK28;
+        acc >>= 28;
+    }
+    acc += (int32_t)a.v[NL - 1] - (int32_t)b.v[NL - 1] - 2 * (int32_t)c.v[NL - 1] + (int32_t)(6u * PL[NL - 1]);
+    fold_est<0, 3>(r, t, acc);
+}
+
+// r = 3a for a in [0, 2p), in [0, 2p): 3a lies in [0, 6p), tripled limb by
+// limb with the carries, then one fold pass.
+__device__ __forceinline__ void fp_triple(Fp& r, const Fp& a) {
+    uint32_t t[NL];
+    int32_t acc = 0;
+#pragma unroll
+    for (int i = 0; i < NL - 1; ++i) {
+        acc += 3 * (int32_t)a.v[i];
+        t[i] = (uint32_t)acc & MASK28;
+        acc >>= 28;
+    }
+    acc += 3 * (int32_t)a.v[NL - 1];
+    fold_est<0, 2>(r, t, acc);
+}
+
 // Brings a reduced value in (-4p, 4p) (28-bit limbs 0..14 and a signed top
 // limb) to [0, 2p), in one pass (fold_est).
 __device__ __forceinline__ void redc_fold_wide(Fp& r, uint32_t (&t)[NL], int64_t top) {

@@ -32,8 +32,8 @@ struct Split<Fp2QuadOps> {
 
 // r = 2p (dbl-2008-s-1), four split steps:
 //   1: V = U^2 | T = X^2            (U = 2Y)
-//   2: W = U V | S = X V            (M = 3T)
-//   3: M^2     | ZZ3 = V ZZ         (X3 = M^2 - 2S)
+//   2: W = U V | S = X V            (M = 3T, one fold pass)
+//   3: M^2     | ZZ3 = V ZZ         (X3 = M^2 - 2S, one fold pass)
 //   4: Y3 = M (S - X3) - W Y | ZZZ3 = W ZZZ   (both as a difference of products)
 template <class G>
 __device__ __forceinline__ void split_dbl(Xyzz<G>& r, const Xyzz<G>& p) {
@@ -51,16 +51,14 @@ __device__ __forceinline__ void split_dbl(Xyzz<G>& r, const Xyzz<G>& p) {
     fp_sel(a, h, p.X, U);
     O::mul(u, a, V);
     Split<G>::results(W, S, u);
-    O::dbl(M, T);
-    O::add(M, M, T);
+    fp_triple(M, T);
     fp_sel(a, h, V, M);
     fp_sel(b, h, p.ZZ, M);
     O::mul(u, a, b);
     Split<G>::results(MM, Z2, u);
-    O::sub(X3, MM, S);
-    O::sub(X3, X3, S);
-    O::sub_raw(d, S, X3);  // a product operand only
     O::zero(zero);
+    fp_sub_b2c(X3, MM, zero, S);
+    O::sub_raw(d, S, X3);  // a product operand only
     fp_sel(a, h, W, M);
     fp_sel(b, h, p.ZZZ, d);
     fp_sel(c, h, zero, W);
@@ -78,7 +76,7 @@ __device__ __forceinline__ void split_dbl(Xyzz<G>& r, const Xyzz<G>& p) {
 //   2: S1 = Y1 ZZZ2  | S2 = Y2 ZZZ1      (R = S2 - S1)
 //   3: PP = P^2      | RR = R^2
 //   4: PPP = PP P    | Q = U1 PP
-//   5: ZZ1 ZZ2       | ZZZ1 ZZZ2         (X3 = RR - PPP - 2Q)
+//   5: ZZ1 ZZ2       | ZZZ1 ZZZ2         (X3 = RR - PPP - 2Q, one fold pass)
 //   6: ZZ3 = ZZ1 ZZ2 PP | ZZZ3 = ZZZ1 ZZZ2 PPP
 //   7: R (Q - X3)    | S1 PPP            (Y3 = difference of the two)
 // r may alias p or q: every input is read before r is written.
@@ -123,9 +121,7 @@ __device__ __forceinline__ void split_add(Xyzz<G>& r, const Xyzz<G>& p, const Xy
     fp_sel(b, h, q.ZZZ, q.ZZ);
     O::mul(u, a, b);
     Split<G>::results(T1, T2, u);
-    O::sub(X3, RR, PPP);
-    O::sub(X3, X3, Q);
-    O::sub(X3, X3, Q);
+    fp_sub_b2c(X3, RR, PPP, Q);
     fp_sel(a, h, T2, T1);
     fp_sel(b, h, PPP, PP);
     O::mul(u, a, b);
