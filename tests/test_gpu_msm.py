@@ -147,6 +147,25 @@ def test_edge_cases_g1():
     assert as_point("g1", r.to_affine().rec[0]) == m.ec_mul(m.FP_OPS, m.G1_STD, 1 << 310)
 
 
+def test_edge_cases_g2_doubling_and_cancellation():
+    """The G2 accumulation's own special branches (two lanes per Fp2 element):
+    P + (-P) in one bucket gives O, equal points take the doubling path, and a
+    bucket that returns to O mid-run continues from there."""
+    g = o.g2_generator()
+    ng = o.scalar_mul("g2", g, m.R_ORDER - 1)
+    two = o.scalar_mul("g2", g, 2)
+    pts = np.concatenate([g, ng])
+    r = gm.G2Projective.msm_multi(pts, np.array([[5], [5]], dtype=np.uint64), montgomery=True).rec
+    assert not r.view(np.uint64).any()
+    pts = np.concatenate([g, g, g, two])
+    r = gm.G2Projective.msm_multi(pts, np.array([[1], [1], [1], [3]], dtype=np.uint64), montgomery=True)
+    assert as_point("g2", r.to_affine().rec[0]) == m.ec_mul(m.FP2_OPS, m.G2_STD, 9)
+    # g - g cancels, then 2g + g + g: 4g per unit of scalar
+    pts = np.concatenate([g, ng, two, g, g])
+    r = gm.G2Projective.msm_multi(pts, np.full((5, 1), 7, dtype=np.uint64), montgomery=True)
+    assert as_point("g2", r.to_affine().rec[0]) == m.ec_mul(m.FP2_OPS, m.G2_STD, 28)
+
+
 def test_edge_cases_g2_infinity_and_zero():
     ks = o.gen_u64(81, 40)
     bases = o.bases_from_k("g2", ks)
