@@ -503,6 +503,31 @@ def test_heavy_buckets_repeated_scalars_and_tiny_top_window():
 
 
 @pytest.mark.parametrize("curve", ["g1", "g2"])
+def test_equal_and_cancelling_chunk_partials(curve):
+    """Every base the same point and every scalar the same: each window's
+    entries fill one bucket, every chunk partial is the same multiple of P, and
+    the fix-up's sums double (heavy-bucket tree and serial fix-up alike).
+    Then P and -P alternate under equal scalars: every even chunk's partial is
+    O, and the MSM is O."""
+    cv = gm.CURVE_G1 if curve == "g1" else gm.CURVE_G2
+    ops, gen = (m.FP_OPS, m.G1_STD) if curve == "g1" else (m.FP2_OPS, m.G2_STD)
+    n, k = 1 << 13, 12345
+    b = gm.Bases.from_k(cv, np.full(n, k, dtype=np.uint64))
+    s_val = m.gen_scalars_modr(77, 1)[0]
+    s = gm.Scalars.upload(np.tile(o.int_to_limbs(s_val, 5), (n, 1)))
+    want = m.ec_mul(ops, gen, n * k * s_val % m.R_ORDER)
+    for c in (0, 9, 13):
+        r = gm.msm_resident(b, s, scalar_bits=299, window=c)
+        assert as_point(curve, r[0]) == want, c
+    g = o.g1_generator() if curve == "g1" else o.g2_generator()
+    p = o.scalar_mul(curve, g, k)
+    pts = np.concatenate([p, o.scalar_mul(curve, p, m.R_ORDER - 1)] * (n // 2))
+    proj = gm.G1Projective if curve == "g1" else gm.G2Projective
+    r = proj.msm_multi(pts, np.full((n, 1), 0x9E3779B97F4A7C15, dtype=np.uint64), montgomery=True).rec
+    assert not r.view(np.uint64).any()
+
+
+@pytest.mark.parametrize("curve", ["g1", "g2"])
 def test_global_sort_path_wide_windows(curve):
     """Windows wider than 16 bits place entries with one global atomic per
     entry (the sort's fallback path) instead of LDS tile counters: windows of
