@@ -4,7 +4,8 @@
 // item, so every launch has Stor<K>::LPI threads per item), the accumulation
 // with AccOps<O> (for G2 the same lane pairs, single-chain column sums); a
 // plan with tail_split runs the bucket reduction and window sums with
-// SplitOps<O> (FpPairOps for G1, Fp2QuadOps for G2).
+// SplitOps<O> (FpPairOps for G1, Fp2QuadOps for G2), one with fixup_split
+// the fix-up's serial sums.
 #pragma once
 #include "engine_impl.hpp"
 #include "msm_kernels.hpp"
@@ -36,9 +37,8 @@ hipError_t launch_accumulate(hipStream_t s, const Plan& p, const Aff<O>* d_bases
     return hipGetLastError();
 }
 
-template <class O>
-hipError_t launch_fixup(hipStream_t s, const Plan& p, uint8_t* ws) {
-    using K = typename KernelOps<O>::type;
+template <class K>
+hipError_t launch_fixup_k(hipStream_t s, const Plan& p, uint8_t* ws) {
     using X = typename Stor<K>::X;
     const int nbk = (int)p.bkt_total;
     const uint32_t* off = reinterpret_cast<const uint32_t*>(ws + p.o_off);
@@ -51,8 +51,23 @@ hipError_t launch_fixup(hipStream_t s, const Plan& p, uint8_t* ws) {
     const MsmDesc* desc = reinterpret_cast<const MsmDesc*>(ws + p.o_desc);
     k_bucket_fixup<K><<<blocks_for((long)nbk * Stor<K>::LPI, 64), 64, 0, s>>>(desc, p.count, nbk, off, head, tail,
                                                                              buckets, heavy, heavy_count, entries);
-    GM_TRY(hipGetLastError());
-    // one workgroup per queued heavy bucket, grid-stride over the queue
+    return hipGetLastError();
+}
+
+template <class O>
+hipError_t launch_fixup(hipStream_t s, const Plan& p, uint8_t* ws) {
+    using K = typename KernelOps<O>::type;
+    using X = typename Stor<K>::X;
+    GM_TRY(p.fixup_split ? launch_fixup_k<typename SplitOps<O>::type>(s, p, ws) : launch_fixup_k<K>(s, p, ws));
+    const uint32_t* off = reinterpret_cast<const uint32_t*>(ws + p.o_off);
+    const X* head = reinterpret_cast<const X*>(ws + p.o_head);
+    const X* tail = reinterpret_cast<const X*>(ws + p.o_tail);
+    X* buckets = reinterpret_cast<X*>(ws + p.o_buckets);
+    uint32_t* heavy = reinterpret_cast<uint32_t*>(ws + p.o_heavy);
+    uint32_t* heavy_count = reinterpret_cast<uint32_t*>(ws + p.o_err) + 1;
+    const MsmDesc* desc = reinterpret_cast<const MsmDesc*>(ws + p.o_desc);
+    // one workgroup per queued heavy bucket, grid-stride over the queue (unsplit
+    // either way: its threads share a bucket's partials, then a tree in LDS)
     constexpr int HT = 64 * Stor<K>::LPI;
     k_bucket_fixup_heavy<K, HT><<<2048, HT, 0, s>>>(desc, p.count, off, head, tail, buckets, heavy, heavy_count);
     return hipGetLastError();

@@ -110,6 +110,7 @@ struct Plan {
     uint32_t qmax;         // two-pass placement: block slots per sort tile (0: one pass)
     int lanes;      // accumulation lanes (upper bound: all digits non-zero)
     bool tail_split; // bucket reduction and window sums on split lane groups (FpPairOps / Fp2QuadOps)
+    bool fixup_split; // the fix-up's serial sums on split lane groups
     size_t cub_bytes;
     // workspace offsets (bytes)
     size_t o_desc, o_cnt, o_off, o_rank, o_dig, o_tbase, o_blk, o_qbase, o_tmp, o_sorted, o_buckets, o_head, o_tail,

@@ -178,6 +178,14 @@ bool make_plan(Plan& p, MsmDesc* d, int curve, int L, int bits, const MsmSpec* s
     // split tail kernels only while all segments' lane groups fit one round
     // of one wave per SIMD (1024 SIMDs x 64 lanes)
     if (p.tail_split && 2 * group * seg > 64 * 1024) p.tail_split = false;
+    // The fix-up sums each cut bucket's chunk partials serially, one lane (G1)
+    // or lane pair (G2) per bucket. Few buckets leave it below one wave per
+    // SIMD, where one XYZZ addition costs its full latency (~65 us for G1),
+    // so it splits each addition's products over two lane groups while the
+    // split launch fits two rounds of two waves per SIMD (256K threads):
+    // G1 2^12-2^18 3-7% and G2 2^12-2^16 1.5-7% faster per MSM; past it
+    // (G1 and G2 2^19, 2^20) the split costs 1% or nothing.
+    p.fixup_split = bkt * 2 * group <= 256 * 1024;
     p.stile_total = (uint32_t)stile;
     p.lds_B = lds_sort ? maxB : 0;
     // Windows of more than SORT_BLOCK buckets place their entries in two

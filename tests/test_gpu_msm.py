@@ -356,10 +356,12 @@ def test_reduction_and_window_sum_special_cases(curve):
 def test_tail_split_and_unsplit_agree(curve):
     """The bucket reduction and window sums split each point operation's
     products over two lane groups (G1: two lanes; G2: two lane pairs) while
-    the plan's segments fit one round of waves, and run unsplit past it. One
-    MSM alone (25 windows x 512 segments of 4 buckets: 25,600 G1 or 51,200
-    G2 threads) takes the split kernels; the same MSM in a batch of 64
-    (819,200 segments) the unsplit ones. Both equal the oracle."""
+    the plan's segments fit one round of waves, and run unsplit past it; the
+    fix-up splits while its split launch fits 256K threads. One MSM alone
+    (25 windows x 512 segments of 4 buckets: 25,600 G1 or 51,200 G2 threads;
+    51,200 buckets cut into chunks of 4 entries) takes the split kernels; the
+    same MSM in a batch of 64 (819,200 segments, 3.3M buckets) the unsplit
+    ones. Both equal the oracle."""
     pool_n = 1 << 14
     ks, b = _pool(curve, pool_n, 120)
     ss = o.gen_scalars_modr(121, pool_n)
